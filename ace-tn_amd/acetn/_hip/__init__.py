@@ -1,0 +1,257 @@
+"""ctypes boundary to ``libacetn_hip.so`` (the gfx950 engine, C-ABI declared in
+include/acetn_hip.h).
+
+Mirrors the role of acetn/evolution/_extensions/__init__.py:12-35 (extension
+discovery + availability flag), with one deliberate difference: a caller
+that asks for the 'hip' backend on a machine where the library or the GPU is
+missing gets an exception, never a silent CPU path.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libacetn_hip.so")
+
+c_i64 = ctypes.c_int64
+c_vp = ctypes.c_void_p
+c_dbl = ctypes.c_double
+c_size = ctypes.c_size_t
+c_u64 = ctypes.c_uint64
+
+
+class AcetnSite(ctypes.Structure):
+    """acetn_site of include/acetn_hip.h."""
+    _fields_ = [("A", c_vp), ("C", c_vp * 4), ("E", c_vp * 4),
+                ("C_dim", (c_i64 * 2) * 4), ("E_dim", (c_i64 * 2) * 4),
+                ("D", c_i64), ("d", c_i64)]
+
+
+_lib = None
+_load_error = None
+
+
+def _declare(lib):
+    sig = {
+        "acetn_hip_last_error": (ctypes.c_char_p, []),
+        "acetn_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
+        "acetn_hip_quarter_tensor_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int]),
+        "acetn_hip_quarter_tensor": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int, c_vp,
+                                                    c_vp, c_size, c_vp]),
+        "acetn_hip_half_projectors_workspace": (c_size, [ctypes.POINTER(AcetnSite),
+                                                         ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                         c_i64, c_i64]),
+        "acetn_hip_half_projectors": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
+                                                     ctypes.POINTER(AcetnSite), ctypes.c_int, c_i64,
+                                                     c_i64, ctypes.c_int, c_dbl, c_u64, c_u64, c_vp,
+                                                     c_vp, ctypes.POINTER(c_i64), c_vp, c_vp, c_size,
+                                                     c_vp]),
+        "acetn_hip_renormalize_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                     c_i64, c_i64]),
+        "acetn_hip_renormalize_boundary": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                          c_vp, c_vp, c_i64, c_vp, c_vp, c_i64,
+                                                          c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_gaussian_sketch": (ctypes.c_int, [c_vp, c_i64, c_i64, c_u64, c_u64, c_vp]),
+        "acetn_hip_gemm_workspace": (c_size, [c_i64, c_i64, c_i64]),
+        "acetn_hip_gemm": (ctypes.c_int, [c_i64, c_i64, c_i64, ctypes.c_int, c_vp, c_i64,
+                                          ctypes.c_int, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_i64,
+                                          c_vp, c_size, c_vp]),
+        "acetn_hip_eigh_workspace": (c_size, [c_i64]),
+        "acetn_hip_eigh": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_orthonormalize_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_orthonormalize": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_size, c_vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def _try_load():
+    global _lib, _load_error
+    if _lib is not None or _load_error is not None:
+        return
+    if not os.path.exists(LIB_PATH):
+        _load_error = f"{LIB_PATH} not built (run __graft_entry__.build())"
+        return
+    try:
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    except OSError as e:  # pragma: no cover - depends on the runtime
+        _load_error = f"failed to load {LIB_PATH}: {e}"
+
+
+_try_load()
+HIP_AVAILABLE = _lib is not None
+
+
+def lib():
+    """The loaded library; raises if it is not available."""
+    if _lib is None:
+        raise RuntimeError(f"acetn HIP engine unavailable: {_load_error}")
+    return _lib
+
+
+def require_gpu():
+    """Raise unless the HIP library is loaded and a GPU is visible."""
+    lib()
+    if not torch.cuda.is_available():
+        raise RuntimeError("acetn HIP engine needs a visible AMD GPU (torch.cuda.is_available() "
+                           "is False)")
+
+
+def check(status):
+    if status != 0:
+        msg = lib().acetn_hip_last_error().decode()
+        raise RuntimeError(f"acetn HIP engine error {status}: {msg}")
+
+
+def stream_handle(device=None):
+    return c_vp(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    return c_vp(t.data_ptr()) if t is not None else c_vp(0)
+
+
+class Workspace:
+    """Grow-only device scratch owned by torch's caching allocator."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes, device):
+        nbytes = int(nbytes)
+        if self.buf is None or self.buf.numel() < nbytes or self.buf.device != device:
+            self.buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+        return self.buf
+
+
+_ws = Workspace()
+
+
+def workspace(nbytes, device):
+    return _ws.get(nbytes, device)
+
+
+def make_site(A, C, E):
+    """Build an AcetnSite view of device tensors (made contiguous FP64); the
+    returned tuple keeps the tensors alive."""
+    keep = [A.contiguous()] + [c.contiguous() for c in C] + [e.contiguous() for e in E]
+    s = AcetnSite()
+    s.A = keep[0].data_ptr()
+    D = A.shape[0]
+    s.D = D
+    s.d = A.shape[4]
+    for k in range(4):
+        c, e = keep[1 + k], keep[5 + k]
+        s.C[k] = c.data_ptr()
+        s.E[k] = e.data_ptr()
+        s.C_dim[k][0], s.C_dim[k][1] = c.shape[0], c.shape[1]
+        s.E_dim[k][0], s.E_dim[k][1] = e.shape[0], e.shape[1]
+    return s, keep
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(512)
+    check(lib().acetn_hip_device_info(buf, 512))
+    return buf.value.decode()
+
+
+# ---------------------------------------------------------------- wrappers
+def gaussian_sketch(n, q, seed, stream_id, device="cuda"):
+    out = torch.empty(n, q, dtype=torch.float64, device=device)
+    check(lib().acetn_hip_gaussian_sketch(ptr(out), n, q, seed, stream_id, stream_handle()))
+    return out
+
+
+def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None):
+    """Dense FP64 GEMM on the matrix cores: alpha*op(A) op(B) + beta*C."""
+    A = A.contiguous()
+    B = B.contiguous()
+    M = A.shape[1] if ta else A.shape[0]
+    K = A.shape[0] if ta else A.shape[1]
+    N = B.shape[0] if tb else B.shape[1]
+    if C is None:
+        C = torch.zeros(M, N, dtype=torch.float64, device=A.device)
+    nb = lib().acetn_hip_gemm_workspace(M, N, K)
+    ws = workspace(nb, A.device)
+    check(lib().acetn_hip_gemm(M, N, K, int(ta), ptr(A), A.shape[1], int(tb), ptr(B), B.shape[1],
+                               alpha, beta, ptr(C), C.shape[1], ptr(ws), ws.numel(), stream_handle()))
+    return C
+
+
+def eigh(G):
+    """Descending eigen-decomposition of a symmetric FP64 matrix."""
+    q = G.shape[0]
+    G = G.clone().contiguous()
+    w = torch.empty(q, dtype=torch.float64, device=G.device)
+    V = torch.empty(q, q, dtype=torch.float64, device=G.device)
+    nb = lib().acetn_hip_eigh_workspace(q)
+    ws = workspace(nb, G.device)
+    check(lib().acetn_hip_eigh(ptr(G), q, ptr(w), ptr(V), ptr(ws), ws.numel(), stream_handle()))
+    return w, V
+
+
+def orthonormalize(Y):
+    Y = Y.clone().contiguous()
+    m, q = Y.shape
+    nb = lib().acetn_hip_orthonormalize_workspace(m, q)
+    ws = workspace(nb, Y.device)
+    check(lib().acetn_hip_orthonormalize(ptr(Y), m, q, ptr(ws), ws.numel(), stream_handle()))
+    return Y
+
+
+def quarter_tensor(site_tensors, k):
+    """site_tensors: (A, [C0..C3], [E0..E3]) device tensors."""
+    s, keep = make_site(*site_tensors)
+    N1 = s.E_dim[k % 4][1] * s.D * s.D
+    N2 = s.E_dim[(3 + k) % 4][0] * s.D * s.D
+    Q = torch.empty(N1, N2, dtype=torch.float64, device=keep[0].device)
+    nb = lib().acetn_hip_quarter_tensor_workspace(ctypes.byref(s), k)
+    ws = workspace(nb, Q.device)
+    check(lib().acetn_hip_quarter_tensor(ctypes.byref(s), k, ptr(Q), ptr(ws), ws.numel(),
+                                         stream_handle()))
+    return Q
+
+
+def half_projectors(site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id):
+    """Returns (P1, P2, spectrum) with P1/P2 shaped (rows, cD_new)."""
+    s1, keep1 = make_site(*site1)
+    s4, keep4 = make_site(*site4)
+    dev = keep1[0].device
+    rows = s1.E_dim[(3 + k) % 4][0] * s1.D * s1.D
+    P1 = torch.empty(rows * chi, dtype=torch.float64, device=dev)
+    P2 = torch.empty(rows * chi, dtype=torch.float64, device=dev)
+    spec = torch.zeros(chi + oversampling, dtype=torch.float64, device=dev)
+    cD = c_i64(0)
+    nb = lib().acetn_hip_half_projectors_workspace(ctypes.byref(s1), ctypes.byref(s4), k, chi,
+                                                   oversampling)
+    ws = workspace(nb, dev)
+    check(lib().acetn_hip_half_projectors(ctypes.byref(s1), ctypes.byref(s4), k, chi, oversampling,
+                                          niter, cutoff, seed, stream_id, ptr(P1), ptr(P2),
+                                          ctypes.byref(cD), ptr(spec), ptr(ws), ws.numel(),
+                                          stream_handle()))
+    n = cD.value
+    return P1[:rows * n].view(rows, n), P2[:rows * n].view(rows, n), spec
+
+
+def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j):
+    """Returns (C_a, C_b, E_o): the new C[(3+k)%4], C[k], E[(3+k)%4] of s2."""
+    s1, keep = make_site(*site1)
+    dev = keep[0].device
+    D = s1.D
+    cDi, cDj = P1i.shape[1], P1j.shape[1]
+    xa = s1.E_dim[(2 + k) % 4][0]
+    xc = s1.E_dim[k % 4][1]
+    Ca = torch.empty(xa, cDi, dtype=torch.float64, device=dev)
+    Cb = torch.empty(cDj, xc, dtype=torch.float64, device=dev)
+    Eo = torch.empty(cDi, cDj, D, D, dtype=torch.float64, device=dev)
+    P1i, P2i, P1j, P2j = (t.contiguous() for t in (P1i, P2i, P1j, P2j))
+    nb = lib().acetn_hip_renormalize_workspace(ctypes.byref(s1), k, cDi, cDj)
+    ws = workspace(nb, dev)
+    check(lib().acetn_hip_renormalize_boundary(ctypes.byref(s1), k, ptr(P1i), ptr(P2i), cDi,
+                                               ptr(P1j), ptr(P2j), cDj, ptr(Ca), ptr(Cb), ptr(Eo),
+                                               ptr(ws), ws.numel(), stream_handle()))
+    return Ca, Cb, Eo

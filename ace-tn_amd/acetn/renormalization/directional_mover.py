@@ -1,0 +1,89 @@
+"""Directional moves (acetn/renormalization/directional_mover.py:5-366).
+
+The move bookkeeping (which plaquette feeds which projector, which site
+receives which renormalised tensor) is restated here on the host; the
+arithmetic runs in acetn_hip_half_projectors / acetn_hip_renormalize_boundary.
+"""
+from .. import _hip
+from .projectors import ProjectorCalculator
+
+
+class DirectionalMover:
+    def __init__(self, config, sketch=None):
+        self.projector_calculator = ProjectorCalculator(config, sketch)
+        self.calculate_projectors = self.projector_calculator.calculate
+
+    # ---- moves: directional_mover.py:23-97
+    def left_move(self, ipeps, xi):
+        self._move(ipeps, xi, 0)
+
+    def up_move(self, ipeps, yi):
+        self._move(ipeps, yi, 1)
+
+    def right_move(self, ipeps, xi):
+        self._move(ipeps, xi, 2)
+
+    def down_move(self, ipeps, yi):
+        self._move(ipeps, yi, 3)
+
+    def _move(self, ipeps, line, k):
+        n_other = ipeps.ny if k in (0, 2) else ipeps.nx
+        proj1, proj2 = {}, {}
+        for idx in range(n_other):
+            xi, yi = (line, idx) if k in (0, 2) else (idx, line)
+            proj1[idx], proj2[idx] = self.calculate_projectors(
+                ipeps, self.plaquette(ipeps.nx, ipeps.ny, xi, yi, k), k)
+        for idx in range(n_other):
+            s1, s2, i, j = self.targets(ipeps.nx, ipeps.ny, line, k, idx)
+            self.renormalize_boundary(ipeps, proj1, proj2, s1, s2, i, j, k)
+
+    # ---- calculate_{left,up,right,down}_projectors: directional_mover.py:99-181
+    @staticmethod
+    def plaquette(nx, ny, xi, yi, k):
+        if k == 0:
+            xj, yj = (xi + 1) % nx, (yi - 1 + ny) % ny
+            return [(xi, yi), (xj, yi), (xj, yj), (xi, yj)]
+        if k == 2:
+            xj, yj = (xi - 1 + nx) % nx, (yi + 1) % ny
+            return [(xi, yi), (xj, yi), (xj, yj), (xi, yj)]
+        if k == 1:
+            xj, yj = (xi - 1 + nx) % nx, (yi - 1 + ny) % ny
+            return [(xi, yi), (xi, yj), (xj, yj), (xj, yi)]
+        xj, yj = (xi + 1) % nx, (yi + 1) % ny
+        return [(xi, yi), (xi, yj), (xj, yj), (xj, yi)]
+
+    def calculate_left_projectors(self, ipeps, xi, yi):
+        return self.calculate_projectors(ipeps, self.plaquette(ipeps.nx, ipeps.ny, xi, yi, 0), 0)
+
+    def calculate_right_projectors(self, ipeps, xi, yi):
+        return self.calculate_projectors(ipeps, self.plaquette(ipeps.nx, ipeps.ny, xi, yi, 2), 2)
+
+    def calculate_up_projectors(self, ipeps, xi, yi):
+        return self.calculate_projectors(ipeps, self.plaquette(ipeps.nx, ipeps.ny, xi, yi, 1), 1)
+
+    def calculate_down_projectors(self, ipeps, xi, yi):
+        return self.calculate_projectors(ipeps, self.plaquette(ipeps.nx, ipeps.ny, xi, yi, 3), 3)
+
+    @staticmethod
+    def targets(nx, ny, line, k, idx):
+        """(s1, s2, i, j) of each renormalisation in a move."""
+        if k == 0:
+            return (line, idx), ((line + 1) % nx, idx), idx, (idx + 1) % ny
+        if k == 2:
+            return (line, idx), ((line - 1 + nx) % nx, idx), idx, (idx - 1 + ny) % ny
+        if k == 1:
+            return (idx, line), (idx, (line - 1 + ny) % ny), idx, (idx + 1) % nx
+        return (idx, line), (idx, (line + 1) % ny), idx, (idx - 1 + nx) % nx
+
+    def renormalize_boundary(self, ipeps, proj1, proj2, s1, s2, i, j, k):
+        """directional_mover.py:273-303: writes C[(3+k)%4], C[k], E[(3+k)%4]
+        of s2."""
+        st = ipeps[s1]
+        flat = lambda P: P.reshape(-1, P.shape[-1])
+        Ca, Cb, Eo = _hip.renormalize_boundary((st['A'], st['C'], st['E']), k,
+                                               flat(proj1[i]), flat(proj2[i]),
+                                               flat(proj1[j]), flat(proj2[j]))
+        dst = ipeps[s2]
+        dst['C'][(3 + k) % 4] = Ca
+        dst['C'][k] = Cb
+        dst['E'][(3 + k) % 4] = Eo

@@ -1,0 +1,68 @@
+"""Projector construction (acetn/renormalization/projectors.py:6-236) on the
+HIP engine.  The enlarged corners, the randomized range finder, the FP64
+Jacobi SVD and the projector products all run on the GPU
+(acetn_hip_half_projectors)."""
+from .. import _hip
+
+
+class SketchCounter:
+    """Numbers the rSVD calls: call i uses Philox stream (first + i) under
+    ``seed``; the reference draws a fresh torch.randn sketch per call
+    (acetn/linalg/fused_matmul_svd_lowrank.py:32)."""
+
+    def __init__(self, seed, first=0):
+        self.seed = int(seed)
+        self.next = int(first)
+
+    def take(self):
+        s = self.next
+        self.next += 1
+        return s
+
+
+class ProjectorCalculator:
+    def __init__(self, config, sketch=None):
+        self.projectors = config.projectors
+        self.svd_type = config.svd_type
+        self.svd_cutoff = config.svd_cutoff
+        self.rsvd_niter = config.rsvd_niter
+        self.rsvd_oversampling = config.rsvd_oversampling
+        self.sketch = sketch if sketch is not None else SketchCounter(0)
+        self.last_spectrum = None
+        self.set_calculate()
+
+    def set_calculate(self):
+        """projectors.py:21-33."""
+        if self.projectors is None or self.projectors == "full-system":
+            self.calculate = self.calculate_full_system
+        elif self.projectors == "half-system":
+            self.calculate = self.calculate_half_system
+        else:
+            raise ValueError(f"Invalid ctmrg projector type: {self.projectors} provided.")
+
+    @staticmethod
+    def _site(ipeps, s):
+        st = ipeps[s]
+        return st['A'], st['C'], st['E']
+
+    def calculate_half_system(self, ipeps, sites, k):
+        """projectors.py:138-174 (rsvd branch)."""
+        sites = [tuple(s) for s in sites]
+        for s in sites:
+            ipeps[s]  # raises ValueError for an unknown site
+        if self.svd_type != "rsvd":
+            raise NotImplementedError("svd_type='full-rank' is not provided by the HIP engine; "
+                                      "use 'rsvd'")
+        s1, s4 = sites[0], sites[3]
+        chi = ipeps.dims['chi']
+        P1, P2, spec = _hip.half_projectors(self._site(ipeps, s1), self._site(ipeps, s4), k % 4,
+                                            chi, self.rsvd_oversampling, self.rsvd_niter,
+                                            self.svd_cutoff, self.sketch.seed, self.sketch.take())
+        self.last_spectrum = spec
+        D = ipeps.dims['bond']
+        rows = P1.shape[0]
+        return P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)
+
+    def calculate_full_system(self, ipeps, sites, k):
+        raise NotImplementedError("projectors='full-system' is not provided by the HIP engine yet; "
+                                  "use 'half-system'")

@@ -1,0 +1,547 @@
+// CTMRG engine: host-side orchestration of the HIP kernels for one projector
+// pair / one boundary renormalisation, plus the extern "C" boundary.
+//
+// Every operation is written once against a planning context ``Ctx``: a first
+// pass in "measure" mode only sizes the scratch buffers (the *_workspace
+// queries), the second pass carves them out of the caller's workspace and
+// enqueues the kernels on the caller's stream.
+#include "internal.h"
+#include "../../include/acetn_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace acetn {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Ctx {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+  bool run = false;
+  hipStream_t st = nullptr;
+  double* gwork = nullptr;
+  size_t gwork_elems = 0;
+  size_t gwork_need = 0;
+
+  double* take(size_t n) {
+    const size_t bytes = ((n * sizeof(double) + 255) / 256) * 256;
+    if (!run) { off += bytes; return nullptr; }
+    ACETN_REQUIRE(off + bytes <= cap, "workspace too small");
+    double* p = reinterpret_cast<double*>(base + off);
+    off += bytes;
+    return p;
+  }
+  int* take_int(size_t n) { return reinterpret_cast<int*>(take((n + 1) / 2)); }
+};
+
+// Run ``body`` in measure mode and return the bytes it needs (gemm split-K
+// scratch first, then the carved buffers).
+template <class F>
+size_t measure(F&& body) {
+  Ctx c;
+  body(c);
+  return c.off + ((c.gwork_need * sizeof(double) + 255) / 256) * 256 + 256;
+}
+
+template <class F>
+void execute(void* ws, size_t ws_bytes, void* stream, F&& body) {
+  Ctx m;
+  body(m);  // sizes
+  Ctx c;
+  c.run = true;
+  c.st = reinterpret_cast<hipStream_t>(stream);
+  uintptr_t b = reinterpret_cast<uintptr_t>(ws);
+  uintptr_t aligned = (b + 255) & ~uintptr_t(255);
+  const size_t lost = aligned - b;
+  ACETN_REQUIRE(ws != nullptr || (m.off == 0 && m.gwork_need == 0), "workspace is null");
+  ACETN_REQUIRE(ws_bytes >= lost, "workspace too small");
+  c.base = reinterpret_cast<char*>(aligned);
+  c.cap = ws_bytes - lost;
+  c.gwork_elems = m.gwork_need;
+  if (m.gwork_need) c.gwork = c.take(m.gwork_need);
+  body(c);
+}
+
+void gemm(Ctx& c, const GemmDesc& g, const double* A, const double* B, double* C) {
+  if (!c.run) {
+    c.gwork_need = std::max(c.gwork_need, gemm_f64_work_elems(g, 1));
+    return;
+  }
+  GemmBatch b{};
+  b.count = 1;
+  b.A[0] = A; b.B[0] = B; b.C[0] = C;
+  gemm_f64(g, b, c.gwork, c.gwork_elems, c.st);
+}
+
+// dense row-major helpers
+void mm(Ctx& c, int M, int N, int K, bool ta, const double* A, int64_t lda, bool tb,
+        const double* B, int64_t ldb, double* C, int64_t ldc, double alpha = 1.0,
+        double beta = 0.0) {
+  gemm(c, dense_desc(M, N, K, ta, lda, tb, ldb, ldc, alpha, beta), A, B, C);
+}
+
+void check_site(const acetn_site* s) {
+  ACETN_REQUIRE(s != nullptr && s->A != nullptr, "site is null");
+  ACETN_REQUIRE(s->D >= 1 && s->D <= 8 && s->d >= 1 && s->d <= 8, "unsupported D/d");
+  for (int k = 0; k < 4; ++k) {
+    ACETN_REQUIRE(s->C[k] && s->E[k], "site tensor pointer is null");
+    ACETN_REQUIRE(s->C_dim[k][0] > 0 && s->C_dim[k][1] > 0 && s->E_dim[k][0] > 0 &&
+                      s->E_dim[k][1] > 0, "non-positive boundary dimension");
+  }
+}
+
+// strides of bond_permute(A, k) axis j (A contiguous (D,D,D,D,d))
+void permuted_strides(const acetn_site* s, int k, int64_t out[4]) {
+  const int64_t D = s->D, d = s->d;
+  const int64_t as[4] = {D * D * D * d, D * D * d, D * d, d};
+  for (int j = 0; j < 4; ++j) out[j] = as[(j + k) % 4];
+}
+
+// ---------------------------------------------------------------------------
+// make_quarter_tensor (acetn/renormalization/projectors.py:35-60)
+//   qk = C[k] . E[k]          "ab,bcuU->acuU"
+//   qk = qk . E[(3+k)%4]      "acuU,ealL->cuUelL"
+//   double layer with A_k, conj(A_k) -> "crRedD"
+// ---------------------------------------------------------------------------
+struct QuarterDims {
+  int64_t ca, cb, cc, ce, D, N1, N2;
+};
+QuarterDims quarter_dims(const acetn_site* s, int k) {
+  k &= 3;
+  const int64_t D = s->D;
+  QuarterDims q{};
+  q.ca = s->C_dim[k][0];
+  q.cb = s->C_dim[k][1];
+  ACETN_REQUIRE(s->E_dim[k][0] == q.cb, "quarter: C[k] / E[k] dimension mismatch");
+  q.cc = s->E_dim[k][1];
+  const int k1 = (3 + k) & 3;
+  ACETN_REQUIRE(s->E_dim[k1][1] == q.ca, "quarter: C[k] / E[k+3] dimension mismatch");
+  q.ce = s->E_dim[k1][0];
+  q.D = D;
+  q.N1 = q.cc * D * D;
+  q.N2 = q.ce * D * D;
+  return q;
+}
+
+void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
+  const QuarterDims qd = quarter_dims(s, k);
+  const int kk = k & 3;
+  const int64_t D = qd.D, D2 = D * D;
+  double* T1 = c.take((size_t)qd.ca * qd.cc * D2);
+  double* T2 = c.take((size_t)qd.N1 * qd.N2);
+  // T1[a, (c,u,U)] = sum_b C[a,b] E[k][b,(c,u,U)]
+  mm(c, (int)qd.ca, (int)(qd.cc * D2), (int)qd.cb, false, s->C[kk], qd.cb, false, s->E[kk],
+     qd.cc * D2, T1, qd.cc * D2);
+  // T2[(c,u,U),(e,l,L)] = sum_a T1[a,(c,u,U)] E[k+3][e,a,(l,L)]
+  GemmDesc g{};
+  g.M = (int)qd.N1; g.N = (int)qd.N2; g.K = (int)qd.ca;
+  g.am = plain(1); g.ak = plain(qd.N1);
+  g.bk = plain(D2); g.bn = grouped((int)D2, 1, qd.ca * D2);
+  g.cm = plain(qd.N2); g.cn = plain(1);
+  g.alpha = 1.0; g.beta = 0.0;
+  gemm(c, g, T1, s->E[(3 + kk) & 3], T2);
+  if (!c.run) return;
+  int64_t ps[4];
+  permuted_strides(s, kk, ps);
+  DLDesc dl{};
+  dl.D = (int)D; dl.d = (int)s->d; dl.n1 = (int)qd.cc; dl.n2 = (int)qd.ce;
+  // X = T2 viewed as [c][u][U][e][l][L]
+  dl.x_o1 = D2 * qd.N2; dl.x_k1 = D * qd.N2; dl.x_b1 = qd.N2;
+  dl.x_o2 = D2; dl.x_k2 = D; dl.x_b2 = 1;
+  // legs: 1 = u (axis 1 of A_k), 2 = l (axis 0), 3 = r (axis 2), 4 = d (axis 3)
+  dl.a_leg[0] = ps[1]; dl.a_leg[1] = ps[0]; dl.a_leg[2] = ps[2]; dl.a_leg[3] = ps[3];
+  dl.a_p = 1;
+  // Y = Q viewed as [c][r][R][e][d][D]
+  dl.y_o1 = D2 * qd.N2; dl.y_k3 = D * qd.N2; dl.y_b3 = qd.N2;
+  dl.y_o2 = D2; dl.y_k4 = D; dl.y_b4 = 1;
+  double_layer_f64(dl, T2, s->A, Q, c.st);
+}
+
+// ---------------------------------------------------------------------------
+// Orthonormal basis of range(Y), Y (m x q) row-major: shifted CholeskyQR3
+// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020) with rank guard.
+// Result is written back into Y.
+// ---------------------------------------------------------------------------
+void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
+  double* G = c.take((size_t)q * q);
+  double* Rinv = c.take((size_t)q * q);
+  double* Yt = c.take((size_t)m * q);
+  int* dead = c.take_int((size_t)q + 2);
+  const size_t pb = (size_t)q * (q + 1) / 2;
+  double* scr = (pb * 8 > 150 * 1024) ? c.take(pb) : nullptr;
+  const double u = 1.1102230246251565e-16;
+  // pass 0: shifted; pass 1: plain with rank guard (dead columns -> refilled);
+  // passes 2, 3: plain, orthogonalising the refilled columns.
+  for (int pass = 0; pass < 4; ++pass) {
+    double* src = (pass % 2 == 0) ? Y : Yt;
+    double* dst = (pass % 2 == 0) ? Yt : Y;
+    mm(c, (int)q, (int)q, (int)m, true, src, q, false, src, q, G, q);
+    const double shift = (pass == 0) ? 11.0 * ((double)m * q + (double)q * (q + 1)) * u : 0.0;
+    if (c.run) chol_rinv(G, Rinv, (int)q, shift, pass == 0 ? 0.0 : 1e-14, c.st, scr,
+                         pass == 1 ? dead : nullptr);
+    mm(c, (int)m, (int)q, (int)q, false, src, q, false, Rinv, q, dst, q);
+    if (pass == 1 && c.run) refill_dead_columns(dst, m, (int)q, dead, 0x243F6A8885A308D3ull, c.st);
+  }
+  // four passes leave the result in Y
+}
+
+// ---------------------------------------------------------------------------
+// Singular values / left vectors of B (q x n) given as BT = B^T (n x q):
+// two passes of Gram + Jacobi (the second on the rotated, nearly diagonal
+// Gram matrix restores absolute accuracy eps*s0 for the small values).
+// Outputs: w (q, descending eigenvalues of B B^T), UB (q x q).
+// ---------------------------------------------------------------------------
+void small_svd(Ctx& c, const double* BT, int64_t n, int64_t q, double* w, double* UB) {
+  double* G = c.take((size_t)q * q);
+  double* W1 = c.take((size_t)q * q);
+  double* W2 = c.take((size_t)q * q);
+  double* w1 = c.take((size_t)q);
+  double* B2T = c.take((size_t)n * q);
+  double* rot = c.take(jacobi_rot_elems((int)q));
+  mm(c, (int)q, (int)q, (int)n, true, BT, q, false, BT, q, G, q);
+  if (c.run) jacobi_eigh(G, w1, W1, rot, (int)q, c.st);
+  mm(c, (int)n, (int)q, (int)q, false, BT, q, false, W1, q, B2T, q);
+  mm(c, (int)q, (int)q, (int)n, true, B2T, q, false, B2T, q, G, q);
+  if (c.run) jacobi_eigh(G, w, W2, rot, (int)q, c.st);
+  mm(c, (int)q, (int)q, (int)q, false, W1, q, false, W2, q, UB, q);
+}
+
+// ---------------------------------------------------------------------------
+// calculate_half_system, rsvd branch (projectors.py:138-174) with
+// fused_matmul_svd_lowrank (linalg/fused_matmul_svd_lowrank.py:4-52).
+// ---------------------------------------------------------------------------
+struct HalfOut {
+  double* P1;
+  double* P2;
+  int64_t* host_cD;
+  double* spectrum;
+};
+
+void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
+                     int64_t oversampling, int niter, double cutoff, uint64_t seed,
+                     uint64_t sstream, const HalfOut& out) {
+  const QuarterDims d1 = quarter_dims(s1, k);
+  const QuarterDims d4 = quarter_dims(s4, k + 3);
+  ACETN_REQUIRE(d1.N2 == d4.N1, "half system: Q1 columns != Q4 rows");
+  const int64_t m = d1.N1, kk = d1.N2, n = d4.N2;
+  int64_t q = std::min<int64_t>(chi + oversampling, std::min(m, n));
+  ACETN_REQUIRE(q >= 1 && q <= 510, "half system: sketch width out of range");
+  double* Q1 = c.take((size_t)m * kk);
+  double* Q4 = c.take((size_t)kk * n);
+  double* Om = c.take((size_t)n * q);
+  double* Y = c.take((size_t)m * q);
+  double* W = c.take((size_t)kk * q);
+  double* Z = c.take((size_t)n * q);
+  double* Wq = c.take((size_t)kk * q);
+  double* BT = c.take((size_t)n * q);
+  double* w = c.take((size_t)q);
+  double* UB = c.take((size_t)q * q);
+  double* s = c.take((size_t)q);
+  int* cD_dev = c.take_int(2);
+  double* X = c.take((size_t)q * q);
+  double* Vp = c.take((size_t)n * q);
+
+  quarter_tensor(c, s1, k, Q1);
+  quarter_tensor(c, s4, k + 3, Q4);
+  if (c.run) gaussian_sketch(Om, n, q, seed, sstream, c.st);
+  // Y = Q1 (Q4 Omega)
+  mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Om, q, W, q);
+  mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);
+  for (int it = 0; it < niter; ++it) {
+    orthonormalize(c, Y, m, q);
+    mm(c, (int)kk, (int)q, (int)m, true, Q1, kk, false, Y, q, W, q);   // Q1^H Y
+    mm(c, (int)n, (int)q, (int)kk, true, Q4, n, false, W, q, Z, q);    // Q4^H (.)
+    mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Z, q, W, q);   // Q4 (.)
+    mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);  // Q1 (.)
+  }
+  orthonormalize(c, Y, m, q);
+  // B^T = Q4^H (Q1^H Qm): n x q ; keep Wq = Q1^H Qm for P1
+  mm(c, (int)kk, (int)q, (int)m, true, Q1, kk, false, Y, q, Wq, q);
+  mm(c, (int)n, (int)q, (int)kk, true, Q4, n, false, Wq, q, BT, q);
+  small_svd(c, BT, n, q, w, UB);
+  if (!c.run) {
+    // size the split-K scratch of the post-truncation products at cD = chi
+    const int cDp = (int)std::min<int64_t>(chi, q);
+    mm(c, (int)kk, cDp, (int)q, false, nullptr, q, false, nullptr, cDp, nullptr, cDp);
+    mm(c, (int)n, cDp, (int)q, false, nullptr, q, false, nullptr, cDp, nullptr, cDp);
+    mm(c, (int)kk, cDp, (int)n, false, nullptr, n, false, nullptr, cDp, nullptr, cDp);
+    return;
+  }
+  count_kept(w, (int)q, (int)chi, cutoff, s, cD_dev, c.st);
+  int cD = 0;
+  ACETN_HIP_CHECK(hipMemcpyAsync(&cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+  ACETN_HIP_CHECK(hipStreamSynchronize(c.st));
+  ACETN_REQUIRE(cD >= 1, "half system: no singular value above the cutoff");
+  *out.host_cD = cD;
+  if (out.spectrum) ACETN_HIP_CHECK(hipMemcpyAsync(out.spectrum, s, sizeof(double) * q, hipMemcpyDeviceToDevice, c.st));
+  // P1 = Q1^H U S_n^-1/2 = Wq (UB S_n^-1/2)
+  scale_columns(UB, (int)q, s, cD, 0, X, c.st);
+  mm(c, (int)kk, cD, (int)q, false, Wq, q, false, X, cD, out.P1, cD);
+  // P2 = Q4 V S_n^-1/2,  V = B^T UB S^-1
+  scale_columns(UB, (int)q, s, cD, 1, X, c.st);
+  mm(c, (int)n, cD, (int)q, false, BT, q, false, X, cD, Vp, cD);
+  mm(c, (int)kk, cD, (int)n, false, Q4, n, false, Vp, cD, out.P2, cD);
+  fix_projector_gauge(out.P1, out.P2, kk, kk, cD, c.st);
+}
+
+// ---------------------------------------------------------------------------
+// renormalize_boundary (directional_mover.py:273-366)
+// ---------------------------------------------------------------------------
+void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const double* P2i,
+                 int64_t cDi, const double* P1j, const double* P2j, int64_t cDj, double* Ca,
+                 double* Cb, double* Eo) {
+  k &= 3;
+  const int64_t D = s1->D, D2 = D * D;
+  double* nrm = c.take(1024);
+  // ---- cj1: C[(3+k)] with E[(2+k)] and P1_i
+  {
+    const int ke = (2 + k) & 3, kc = (3 + k) & 3;
+    const int64_t xa = s1->E_dim[ke][0], xb = s1->E_dim[ke][1];
+    ACETN_REQUIRE(s1->C_dim[kc][0] == xb, "cj1: E/C dimension mismatch");
+    const int64_t xc = s1->C_dim[kc][1];
+    double* T = c.take((size_t)xa * D2 * xc);
+    GemmDesc g{};
+    g.M = (int)(xa * D2); g.N = (int)xc; g.K = (int)xb;
+    g.am = grouped((int)D2, 1, xb * D2); g.ak = plain(D2);
+    g.bk = plain(xc); g.bn = plain(1);
+    g.cm = plain(xc); g.cn = plain(1);
+    g.alpha = 1.0; g.beta = 0.0;
+    gemm(c, g, s1->E[ke], s1->C[kc], T);
+    GemmDesc h{};
+    h.M = (int)xa; h.N = (int)cDi; h.K = (int)(xc * D2);
+    h.am = plain(D2 * xc); h.ak = grouped((int)D2, xc, 1);
+    h.bk = plain(cDi); h.bn = plain(1);
+    h.cm = plain(cDi); h.cn = plain(1);
+    h.alpha = 1.0; h.beta = 0.0;
+    gemm(c, h, T, P1i, Ca);
+    if (c.run) frobenius_normalize(Ca, xa * cDi, nrm, c.st);
+  }
+  // ---- cj2: C[k] with E[k] and P2_j
+  {
+    const int64_t xa = s1->C_dim[k][0], xb = s1->C_dim[k][1];
+    ACETN_REQUIRE(s1->E_dim[k][0] == xb, "cj2: C/E dimension mismatch");
+    const int64_t xc = s1->E_dim[k][1];
+    double* T = c.take((size_t)xa * xc * D2);
+    mm(c, (int)xa, (int)(xc * D2), (int)xb, false, s1->C[k], xb, false, s1->E[k], xc * D2, T,
+       xc * D2);
+    GemmDesc h{};
+    h.M = (int)cDj; h.N = (int)xc; h.K = (int)(xa * D2);
+    h.am = plain(1); h.ak = plain(cDj);
+    h.bk = grouped((int)D2, 1, xc * D2); h.bn = plain(D2);
+    h.cm = plain(xc); h.cn = plain(1);
+    h.alpha = 1.0; h.beta = 0.0;
+    gemm(c, h, P2j, T, Cb);
+    if (c.run) frobenius_normalize(Cb, cDj * xc, nrm, c.st);
+  }
+  // ---- ej: E[(3+k)] with A_k, P1_j, P2_i
+  {
+    const int ke = (3 + k) & 3;
+    const int64_t xa = s1->E_dim[ke][0], xb = s1->E_dim[ke][1];
+    double* T = c.take((size_t)xa * D2 * D2 * cDj);
+    double* Zt = c.take((size_t)cDj * D2 * xa * D2);
+    GemmDesc g{};
+    g.M = (int)(xa * D2); g.N = (int)(D2 * cDj); g.K = (int)xb;
+    g.am = grouped((int)D2, 1, xb * D2); g.ak = plain(D2);
+    g.bk = plain(D2 * cDj); g.bn = plain(1);
+    g.cm = plain(D2 * cDj); g.cn = plain(1);
+    g.alpha = 1.0; g.beta = 0.0;
+    gemm(c, g, s1->E[ke], P1j, T);
+    if (c.run) {
+      int64_t ps[4];
+      permuted_strides(s1, k, ps);
+      DLDesc dl{};
+      dl.D = (int)D; dl.d = (int)s1->d; dl.n1 = (int)xa; dl.n2 = (int)cDj;
+      const int64_t ld = D2 * cDj;  // row length of T
+      dl.x_o1 = D2 * ld; dl.x_k1 = D * ld; dl.x_b1 = ld;
+      dl.x_k2 = D * cDj; dl.x_b2 = cDj; dl.x_o2 = 1;
+      dl.a_leg[0] = ps[0]; dl.a_leg[1] = ps[1]; dl.a_leg[2] = ps[2]; dl.a_leg[3] = ps[3];
+      dl.a_p = 1;
+      // Z[x][r][R][a][d][D]
+      const int64_t rowz = xa * D2;
+      dl.y_o2 = D2 * rowz; dl.y_k3 = D * rowz; dl.y_b3 = rowz;
+      dl.y_o1 = D2; dl.y_k4 = D; dl.y_b4 = 1;
+      double_layer_f64(dl, T, s1->A, Zt, c.st);
+    }
+    GemmDesc h{};
+    h.M = (int)(cDj * D2); h.N = (int)cDi; h.K = (int)(xa * D2);
+    h.am = plain(xa * D2); h.ak = plain(1);
+    h.bk = plain(cDi); h.bn = plain(1);
+    h.cm = plain(1); h.cn = plain(cDj * D2);
+    h.alpha = 1.0; h.beta = 0.0;
+    gemm(c, h, Zt, P2i, Eo);
+    if (c.run) frobenius_normalize(Eo, cDi * cDj * D2, nrm, c.st);
+  }
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 3;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return 3;
+  }
+}
+
+}  // namespace
+}  // namespace acetn
+
+using namespace acetn;
+
+extern "C" {
+
+const char* acetn_hip_last_error(void) { return g_last_error.c_str(); }
+
+int acetn_hip_device_info(char* buf, size_t len) {
+  return guarded([&] {
+    int dev = 0;
+    ACETN_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    ACETN_HIP_CHECK(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, len, "%s|%s|cu=%d|lds=%zu|clock_khz=%d", p.name, p.gcnArchName,
+             p.multiProcessorCount, (size_t)p.sharedMemPerBlock, p.clockRate);
+  });
+}
+
+size_t acetn_hip_quarter_tensor_workspace(const acetn_site* s, int k) {
+  size_t r = 0;
+  guarded([&] { check_site(s); r = measure([&](Ctx& c) { quarter_tensor(c, s, k, nullptr); }); });
+  return r;
+}
+
+int acetn_hip_quarter_tensor(const acetn_site* s, int k, double* Q, void* ws, size_t ws_bytes,
+                             void* stream) {
+  return guarded([&] {
+    check_site(s);
+    execute(ws, ws_bytes, stream, [&](Ctx& c) { quarter_tensor(c, s, k, Q); });
+  });
+}
+
+size_t acetn_hip_half_projectors_workspace(const acetn_site* s1, const acetn_site* s4, int k,
+                                           int64_t chi, int64_t oversampling) {
+  size_t r = 0;
+  guarded([&] {
+    check_site(s1);
+    check_site(s4);
+    HalfOut o{nullptr, nullptr, nullptr, nullptr};
+    r = measure([&](Ctx& c) {
+      half_projectors(c, s1, s4, k, chi, oversampling, 2, 0.0, 0, 0, o);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
+                              int64_t oversampling, int niter, double svd_cutoff, uint64_t seed,
+                              uint64_t sketch_stream, double* P1, double* P2,
+                              int64_t* host_cD_new, double* spectrum, void* ws, size_t ws_bytes,
+                              void* stream) {
+  return guarded([&] {
+    check_site(s1);
+    check_site(s4);
+    ACETN_REQUIRE(P1 && P2 && host_cD_new, "half_projectors: null output");
+    ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
+    HalfOut o{P1, P2, host_cD_new, spectrum};
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o);
+    });
+  });
+}
+
+size_t acetn_hip_renormalize_workspace(const acetn_site* s1, int k, int64_t cD_i, int64_t cD_j) {
+  size_t r = 0;
+  guarded([&] {
+    check_site(s1);
+    r = measure([&](Ctx& c) {
+      renormalize(c, s1, k, nullptr, nullptr, cD_i, nullptr, nullptr, cD_j, nullptr, nullptr,
+                  nullptr);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1_i,
+                                   const double* P2_i, int64_t cD_i, const double* P1_j,
+                                   const double* P2_j, int64_t cD_j, double* C_a, double* C_b,
+                                   double* E_o, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_site(s1);
+    ACETN_REQUIRE(cD_i >= 1 && cD_j >= 1, "renormalize: empty projector");
+    ACETN_REQUIRE(P1_i && P2_i && P1_j && P2_j && C_a && C_b && E_o, "renormalize: null pointer");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o);
+    });
+  });
+}
+
+int acetn_hip_gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed,
+                              uint64_t sketch_stream, void* stream) {
+  return guarded([&] {
+    ACETN_REQUIRE(out && n >= 0 && q >= 0, "sketch: bad arguments");
+    gaussian_sketch(out, n, q, seed, sketch_stream, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+size_t acetn_hip_gemm_workspace(int64_t M, int64_t N, int64_t K) {
+  size_t r = 0;
+  guarded([&] {
+    r = measure([&](Ctx& c) {
+      mm(c, (int)M, (int)N, (int)K, false, nullptr, K, false, nullptr, N, nullptr, N);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_gemm(int64_t M, int64_t N, int64_t K, int ta, const double* A, int64_t lda, int tb,
+                   const double* B, int64_t ldb, double alpha, double beta, double* C,
+                   int64_t ldc, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    ACETN_REQUIRE(M >= 0 && N >= 0 && K >= 0 && M < (1LL << 31) && N < (1LL << 31) &&
+                      K < (1LL << 31), "gemm: bad shape");
+    ACETN_REQUIRE(A && B && C, "gemm: null pointer");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      mm(c, (int)M, (int)N, (int)K, ta != 0, A, lda, tb != 0, B, ldb, C, ldc, alpha, beta);
+    });
+  });
+}
+
+size_t acetn_hip_eigh_workspace(int64_t q) {
+  return sizeof(double) * (jacobi_rot_elems((int)q) + 64) + 512;
+}
+
+int acetn_hip_eigh(double* G, int64_t q, double* w, double* V, void* ws, size_t ws_bytes,
+                   void* stream) {
+  return guarded([&] {
+    ACETN_REQUIRE(q >= 1 && q <= 510, "eigh: q out of range");
+    ACETN_REQUIRE(ws_bytes >= acetn_hip_eigh_workspace(q) - 512, "eigh: workspace too small");
+    jacobi_eigh(G, w, V, reinterpret_cast<double*>(ws), (int)q,
+                reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+size_t acetn_hip_orthonormalize_workspace(int64_t m, int64_t q) {
+  size_t r = 0;
+  guarded([&] { r = measure([&](Ctx& c) { orthonormalize(c, nullptr, m, q); }); });
+  return r;
+}
+
+int acetn_hip_orthonormalize(double* Y, int64_t m, int64_t q, void* ws, size_t ws_bytes,
+                             void* stream) {
+  return guarded([&] {
+    ACETN_REQUIRE(Y && m >= 1 && q >= 1 && q <= 510, "orthonormalize: bad arguments");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) { orthonormalize(c, Y, m, q); });
+  });
+}
+
+}  // extern "C"

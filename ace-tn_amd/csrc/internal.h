@@ -1,0 +1,142 @@
+// Internal declarations shared by the acetn HIP translation units.
+// Nothing here crosses the C-ABI; see include/acetn_hip.h for that.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace acetn {
+
+// ---------------------------------------------------------------------------
+// error handling: every HIP call is checked and turned into an exception that
+// the C-ABI layer converts to a status code + message.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define ACETN_HIP_CHECK(expr)                                                   \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess)                                                       \
+      throw ::acetn::Error(2, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define ACETN_REQUIRE(cond, msg)                                                \
+  do {                                                                          \
+    if (!(cond)) throw ::acetn::Error(1, std::string(msg));                     \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Two-level index map: offset(i) = (i % inner) * s_in + (i / inner) * s_out.
+// inner <= 0 means a plain stride: offset(i) = i * s_in.  This lets a GEMM
+// operand index a grouped einsum mode (e.g. (e, l, L) of an edge tensor laid
+// out as [e][a][l][L]) without materialising the permutation.
+// ---------------------------------------------------------------------------
+struct IdxMap {
+  int32_t inner;
+  int64_t s_in;
+  int64_t s_out;
+};
+inline IdxMap plain(int64_t s) { return IdxMap{0, s, 0}; }
+inline IdxMap grouped(int32_t inner, int64_t s_in, int64_t s_out) {
+  return IdxMap{inner, s_in, s_out};
+}
+
+__device__ __forceinline__ int64_t imap(const IdxMap& m, int32_t i) {
+  if (m.inner <= 0) return (int64_t)i * m.s_in;
+  return (int64_t)(i % m.inner) * m.s_in + (int64_t)(i / m.inner) * m.s_out;
+}
+
+constexpr int kMaxBatch = 16;
+
+// C[b](m,n) = alpha * sum_k A[b](m,k) B[b](k,n) + beta * C[b](m,n)
+struct GemmDesc {
+  int32_t M, N, K;
+  IdxMap am, ak;  // A(m,k) = A[am(m) + ak(k)]
+  IdxMap bk, bn;  // B(k,n) = B[bk(k) + bn(n)]
+  IdxMap cm, cn;  // C(m,n) = C[cm(m) + cn(n)]
+  double alpha, beta;
+};
+
+struct GemmBatch {
+  int32_t count;
+  const double* A[kMaxBatch];
+  const double* B[kMaxBatch];
+  double* C[kMaxBatch];
+};
+
+// FP64 MFMA GEMM.  ``work``/``work_elems`` is an optional scratch area for
+// split-K partial sums (ksplit chosen internally when K is long and the tile
+// grid is small).
+void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_elems,
+              hipStream_t s);
+size_t gemm_f64_work_elems(const GemmDesc& g, int count);
+
+// Convenience: dense row-major C = alpha*op(A) op(B) + beta*C.
+// A is (M x K) if !ta else stored (K x M); lda/ldb/ldc are row strides.
+GemmDesc dense_desc(int M, int N, int K, bool ta, int64_t lda, bool tb, int64_t ldb,
+                    int64_t ldc, double alpha = 1.0, double beta = 0.0);
+
+// ---------------------------------------------------------------------------
+// Double-layer absorption of a site tensor into a block tensor:
+// Y[o1,o2; k3,b3,k4,b4] = sum_{k1,b1,k2,b2,p} X[o1,o2; k1,b1,k2,b2]
+//                           * A[k1,k2,k3,k4,p] * conj(A)[b1,b2,b3,b4,p]
+// A's legs are addressed through strides, which folds bond_permute in.
+// ---------------------------------------------------------------------------
+struct DLDesc {
+  int32_t D, d, n1, n2;
+  int64_t x_o1, x_o2, x_k1, x_b1, x_k2, x_b2;
+  int64_t y_o1, y_o2, y_k3, y_b3, y_k4, y_b4;
+  int64_t a_leg[4];
+  int64_t a_p;
+};
+void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
+                      hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// small dense linear algebra (one workgroup per problem)
+// ---------------------------------------------------------------------------
+// Gaussian sketch (Philox4x32-10 + Box-Muller), n x q row-major.
+void gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed, uint64_t stream,
+                     hipStream_t s);
+
+// Cholesky-based orthonormalisation helper: given the q x q Gram matrix G
+// (row-major, ld=q), write the upper-triangular inverse factor Rinv (q x q,
+// row-major) with G + shift*I = R^T R.  Columns whose pivot falls below
+// rank_tol * max(diag G) are marked dead: their rows/cols of Rinv are zero.
+void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double rank_tol,
+               hipStream_t s, double* gscratch, int* dead_out = nullptr);
+// Replace the columns flagged in ``dead`` (device, q ints) of Y (m x q,
+// row-major) by N(0,1) Philox vectors.
+void refill_dead_columns(double* Y, int64_t m, int q, const int* dead, uint64_t seed,
+                         hipStream_t s);
+
+// Symmetric eigen-decomposition by cyclic parallel Jacobi.  G (q x q,
+// row-major) is overwritten; eigenvalues (descending) go to ``w`` and
+// eigenvectors (columns, row-major q x q) to ``V``.  ``rot`` is scratch for
+// the recorded rotations (see jacobi_rot_elems).
+void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s);
+size_t jacobi_rot_elems(int q);
+
+// Column scaling / projector finishing kernels.
+// X[i, z] = Ub[i, z] * f(s[z]) for z < cD, with f selected by ``mode``:
+//   mode 0: (s[z]/s0)^-1/2          (P1 factor)
+//   mode 1: 1 / (s[z] * sqrt(s[z]/s0)) (P2 factor through V = B^T U S^-1)
+void scale_columns(const double* Ub, int q, const double* s, int cD, int mode, double* X,
+                   hipStream_t st);
+// cD_new = min(chi, #{s/s0 > cutoff}) with s = sqrt(max(w, 0)).
+void count_kept(const double* w, int q, int chi, double cutoff, double* s_out, int* cD_out,
+                hipStream_t st);
+// Sign gauge of the projector pair (shared with the oracle).
+void fix_projector_gauge(double* P1, double* P2, int64_t rows1, int64_t rows2, int cD,
+                         hipStream_t st);
+// x /= ||x||_F over n elements (two-pass, deterministic).
+void frobenius_normalize(double* x, int64_t n, double* scratch, hipStream_t st);
+// out[i] = sum of squares partials; used by tests for norms.
+void zero_fill(double* x, int64_t n, hipStream_t st);
+
+}  // namespace acetn

@@ -1,0 +1,126 @@
+/*
+ * acetn_hip.h -- C-ABI of the MI355X (gfx950) engine for ace-tn's CTMRG step,
+ * full-update bond solve and reduced density matrices.
+ *
+ * Conventions (every entry point):
+ *   - returns 0 on success, nonzero on failure; acetn_hip_last_error() gives
+ *     the message of the last failure on the calling thread;
+ *   - pointers are device pointers (FP64, contiguous row-major) unless the
+ *     parameter name says host;
+ *   - outputs and the scratch ``workspace`` are caller-allocated (the Python
+ *     host side allocates them with torch), sized by the matching
+ *     *_workspace() query;
+ *   - ``stream`` is a hipStream_t passed as void* (torch's current stream);
+ *     work is enqueued on it.  Functions that return a truncation rank to
+ *     the host synchronise that stream, as the reference does at
+ *     acetn/renormalization/projectors.py:164.
+ *
+ * Index conventions follow the reference: a site tensor A is (l,u,r,d,p),
+ * direction k: 0=left 1=up 2=right 3=down (acetn/ipeps/bond.py:11),
+ * corners C[k] are (chi, chi), edges E[k] are (chi, chi, D_ket, D_bra)
+ * (acetn/ipeps/site_tensor.py:152-202).
+ */
+#ifndef ACETN_HIP_H
+#define ACETN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One site of the unit cell as the kernels see it: the tensors of
+ * acetn.ipeps.SiteTensor (acetn/ipeps/site_tensor.py:4-202) with their
+ * current (possibly still growing) boundary dimensions. */
+typedef struct acetn_site {
+  const double* A;      /* (D, D, D, D, d) */
+  const double* C[4];   /* C[k]: (C_dim[k][0], C_dim[k][1]) */
+  const double* E[4];   /* E[k]: (E_dim[k][0], E_dim[k][1], D, D) */
+  int64_t C_dim[4][2];
+  int64_t E_dim[4][2];
+  int64_t D;            /* bond dimension */
+  int64_t d;            /* physical dimension */
+} acetn_site;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* acetn_hip_last_error(void);
+
+/* Library / device self-description: writes up to ``len`` bytes. */
+int acetn_hip_device_info(char* host_buf, size_t len);
+
+/* Enlarged corner ("quarter tensor") Q_k of one site, as an
+ * (C_dim[k][1]*D^2) x (E_dim[(3+k)%4][0]*D^2) matrix.
+ * Replaces ProjectorCalculator.make_quarter_tensor,
+ * acetn/renormalization/projectors.py:35-60 (the max-abs scaling of :59 is a
+ * scale that cancels in the Frobenius-normalised C/E and is not applied). */
+size_t acetn_hip_quarter_tensor_workspace(const acetn_site* s, int k);
+int acetn_hip_quarter_tensor(const acetn_site* s, int k, double* Q, void* workspace,
+                             size_t ws_bytes, void* stream);
+
+/* Half-system projector pair from the randomized SVD of Q1*Q4 (Q1 of site
+ * s1 with rotation k, Q4 of site s4 with rotation k+3).
+ * Replaces calculate_half_system (rsvd branch),
+ * acetn/renormalization/projectors.py:138-174, and fused_matmul_svd_lowrank,
+ * acetn/linalg/fused_matmul_svd_lowrank.py:4-52.  The sketch is the
+ * Philox4x32-10 Gaussian of stream ``sketch_stream`` under ``seed``.
+ * P1, P2: (rows x chi) buffers, written as rows x cD_new (row-major, leading
+ * dimension cD_new).  host_cD_new receives cD_new; spectrum (device, q
+ * doubles) receives the singular values (descending, unnormalised). */
+size_t acetn_hip_half_projectors_workspace(const acetn_site* s1, const acetn_site* s4, int k,
+                                           int64_t chi, int64_t oversampling);
+int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
+                              int64_t oversampling, int niter, double svd_cutoff, uint64_t seed,
+                              uint64_t sketch_stream, double* P1, double* P2,
+                              int64_t* host_cD_new, double* spectrum, void* workspace,
+                              size_t ws_bytes, void* stream);
+
+/* Boundary renormalisation of one (s1 -> s2) target.
+ * Replaces DirectionalMover.renormalize_boundary / renormalize_cj1 /
+ * renormalize_cj2 / renormalize_ej, acetn/renormalization/directional_mover.py:273-366.
+ * Outputs (each Frobenius-normalised):
+ *   C_a = C'[(3+k)%4] of s2: (E_dim[(2+k)%4][0] of s1) x cD_i
+ *   C_b = C'[k] of s2:       cD_j x (E_dim[k][1] of s1)
+ *   E_o = E'[(3+k)%4] of s2: (cD_i, cD_j, D, D)
+ * P1_i/P2_i are the projectors of row/column i, P1_j/P2_j those of j, each
+ * (rows x cD) row-major. */
+size_t acetn_hip_renormalize_workspace(const acetn_site* s1, int k, int64_t cD_i, int64_t cD_j);
+int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1_i,
+                                   const double* P2_i, int64_t cD_i, const double* P1_j,
+                                   const double* P2_j, int64_t cD_j, double* C_a, double* C_b,
+                                   double* E_o, void* workspace, size_t ws_bytes, void* stream);
+
+/* ---- building blocks, exported for the parity tests ---------------------- */
+
+/* Gaussian sketch, n x q row-major (the device side of the generator the
+ * oracle restates; replaces torch.randn at fused_matmul_svd_lowrank.py:32). */
+int acetn_hip_gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed,
+                              uint64_t sketch_stream, void* stream);
+
+/* C = alpha * op(A) op(B) + beta * C, dense row-major FP64 on the matrix
+ * cores (op = transpose when ta/tb != 0).  No reference counterpart: the
+ * reference calls torch matmul/einsum. */
+size_t acetn_hip_gemm_workspace(int64_t M, int64_t N, int64_t K);
+int acetn_hip_gemm(int64_t M, int64_t N, int64_t K, int ta, const double* A, int64_t lda,
+                   int tb, const double* B, int64_t ldb, double alpha, double beta, double* C,
+                   int64_t ldc, void* workspace, size_t ws_bytes, void* stream);
+
+/* Symmetric eigen-decomposition (descending) of a q x q matrix (G is
+ * overwritten).  Replaces torch.linalg.eigh at acetn/evolution/full_update.py:283
+ * and backs the projector SVD. */
+size_t acetn_hip_eigh_workspace(int64_t q);
+int acetn_hip_eigh(double* G, int64_t q, double* w, double* V, void* workspace,
+                   size_t ws_bytes, void* stream);
+
+/* Orthonormal basis of the column space of Y (m x q, row-major, in place),
+ * shifted Cholesky-QR3 with rank guarding.  Replaces torch.linalg.qr at
+ * acetn/linalg/fused_matmul_svd_lowrank.py:35,40. */
+size_t acetn_hip_orthonormalize_workspace(int64_t m, int64_t q);
+int acetn_hip_orthonormalize(double* Y, int64_t m, int64_t q, void* workspace, size_t ws_bytes,
+                             void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACETN_HIP_H */

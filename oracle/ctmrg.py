@@ -8,8 +8,9 @@ Direction k: 0=left, 1=up, 2=right, 3=down (acetn/ipeps/bond.py:11).
 
 Projector gauge: the reference leaves the sign of each singular pair to
 LAPACK.  Both this oracle and the device engine fix it the same way (the
-largest-magnitude entry of each P1 column is made positive, P2 follows);
-this is a pure boundary gauge choice and leaves every observable unchanged.
+weighted column sum sum_i w_i P1[i, z] with w_i = 1 + frac(i*phi) is made
+positive, P2 follows); this is a pure boundary gauge choice and leaves every
+observable unchanged.
 """
 import numpy as np
 
@@ -145,14 +146,22 @@ def svd_lowrank(A, q, niter, omega_fn):
     return Qm @ Ub, S, Vh.conj().T
 
 
+def gauge_weights(n):
+    """w_i = 1 + frac(i * 0.6180339887498949): fixed, positive and asymmetric
+    under any index swap, so the weighted sum below has no symmetry-forced
+    zeros (the ket/bra symmetry of the environment makes max-|entry| rules
+    tie exactly)."""
+    return 1.0 + np.mod(np.arange(n, dtype=np.float64) * 0.6180339887498949, 1.0)
+
+
 def fix_projector_gauge(P1, P2):
     """Sign convention shared with the device engine: for every retained
-    singular index z the largest-|.| entry of P1[..., z] is made positive and
-    P2[..., z] follows.  (Ties are broken by the lowest flat index.)"""
+    singular index z, sign(sum_i w_i P1[i, z]) is made positive and P2[:, z]
+    follows."""
     n = P1.shape[-1]
     f1 = P1.reshape(-1, n)
-    idx = np.argmax(np.abs(f1), axis=0)
-    sgn = np.where(f1[idx, np.arange(n)] < 0, -1.0, 1.0)
+    s = gauge_weights(f1.shape[0]) @ f1
+    sgn = np.where(s < 0, -1.0, 1.0)
     return P1 * sgn, P2 * sgn
 
 

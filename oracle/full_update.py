@@ -4,8 +4,9 @@ Restates acetn/evolution/{tensor_update,full_update,als_solver,
 fast_full_update,evolve,simple_update,lambda_tensors}.py.
 
 Gauge convention shared with the device engine: after
-``finalize_reduced_tensors`` the sign of every retained bond index is chosen so
-that the largest-|.| entry of a1r[:, u, :] is positive (a2r follows).  The
+``finalize_reduced_tensors`` the sign of every retained bond index u is chosen
+so that the weighted sum of a1r[:, u, :] (weights oracle.ctmrg.gauge_weights
+over the flattened (y, p) entries) is positive; a2r follows.  The
 reference leaves this sign to LAPACK; it is a bond gauge and leaves every
 observable unchanged.
 """
@@ -190,11 +191,14 @@ def als_solve(n12, a12g, ar_shape, niter=100, tol=1e-15, method="cholesky",
 
 
 def fix_bond_gauge(a1r, a2r):
-    """Sign convention shared with the device engine (module docstring)."""
+    """Sign convention shared with the device engine (module docstring):
+    sign(sum_i w_i a1r[i, u]) > 0 over the flattened (y, p) entries of each
+    bond index u, with the weights of oracle.ctmrg.gauge_weights."""
+    from .ctmrg import gauge_weights
     nD, bD, pD = a1r.shape
     f = np.transpose(a1r, (1, 0, 2)).reshape(bD, -1)
-    idx = np.argmax(np.abs(f), axis=1)
-    sgn = np.where(f[np.arange(bD), idx] < 0, -1.0, 1.0)
+    s = f @ gauge_weights(f.shape[1])
+    sgn = np.where(s < 0, -1.0, 1.0)
     return a1r * sgn[None, :, None], a2r * sgn[None, :, None]
 
 

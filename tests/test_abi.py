@@ -1,0 +1,52 @@
+"""CPU checks of the drop-in boundary: the library loads and exports every
+symbol include/acetn_hip.h declares (no compute calls without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "acetn_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(acetn_hip_\w+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "acetn_hip_half_projectors" in syms and "acetn_hip_renormalize_boundary" in syms
+    assert len(syms) >= 12
+
+
+def test_library_exports_every_declared_symbol():
+    from acetn import _hip
+    assert os.path.exists(_hip.LIB_PATH), "build the library first (__graft_entry__.build())"
+    lib = ctypes.CDLL(_hip.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_hip_backend_refuses_without_gpu():
+    import torch
+    from acetn import _hip
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError):
+        _hip.require_gpu()
+
+
+def test_config_validation():
+    from acetn.ipeps.ipeps_config import IpepsConfig
+    cfg = IpepsConfig(TN={'nx': 2, 'ny': 2, 'dims': {'phys': 2, 'bond': 2, 'chi': 8}},
+                      evolution={'backend': 'cutensor'}, device="cpu")
+    assert cfg.evolution.backend == "hip"
+    assert cfg.device.type == "cuda"
+    with pytest.raises(ValueError):
+        IpepsConfig(TN={'nx': 2, 'ny': 2}, evolution={'backend': 'torch'})
+    with pytest.raises(TypeError):
+        IpepsConfig(TN={'nx': 2, 'ny': 2}, ctmrg={'no_such_key': 1})
+    with pytest.raises(NotImplementedError):
+        IpepsConfig(TN={'nx': 2, 'ny': 2}, dtype="float32")

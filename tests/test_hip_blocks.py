@@ -1,0 +1,93 @@
+"""Parity of the HIP building blocks (through the C-ABI) against numpy / the
+oracle.  FP64 throughout; tolerances are stated per check."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ctmrg as oc
+from oracle.rng import gaussian_sketch as np_sketch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from acetn import _hip
+    _hip.require_gpu()
+    print("device:", _hip.device_info(), flush=True)
+    return _hip
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def test_sketch_matches_oracle_generator(hip):
+    for n, q, seed, stream in [(7, 3, 0, 0), (1024, 66, 12345, 17), (5184, 146, 2 ** 40 + 3, 2 ** 33 + 5)]:
+        z = hip.gaussian_sketch(n, q, seed, stream).cpu().numpy()
+        ref = np_sketch(n, q, seed, stream)
+        # same Philox bits; Box-Muller transcendental functions may differ in the last ulp
+        np.testing.assert_allclose(z, ref, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(1, 1, 1, False, False), (37, 53, 29, False, False),
+                                         (64, 64, 64, True, False), (130, 70, 300, False, True),
+                                         (100, 146, 5000, True, False), (512, 66, 1024, False, False)])
+def test_gemm_matches_numpy(hip, M, N, K, ta, tb):
+    rng = np.random.default_rng(M * 1000 + N)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = 0.5 * ((A.T if ta else A) @ (B.T if tb else B)) - 2.0 * C0
+    out = hip.gemm(dev(A), dev(B), ta=ta, tb=tb, alpha=0.5, beta=-2.0, C=dev(C0)).cpu().numpy()
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12 * np.sqrt(K))
+
+
+@pytest.mark.parametrize("q", [1, 2, 5, 16, 33, 66, 146, 202])
+def test_eigh_matches_lapack(hip, q):
+    rng = np.random.default_rng(q)
+    X = rng.standard_normal((q, q))
+    G = X @ X.T + np.diag(np.linspace(0, 1, q))
+    w, V = hip.eigh(dev(G))
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    wr = np.linalg.eigvalsh(G)[::-1]
+    np.testing.assert_allclose(w, wr, rtol=1e-12, atol=1e-12 * wr[0])
+    np.testing.assert_allclose(V.T @ V, np.eye(q), atol=1e-12)
+    np.testing.assert_allclose(G @ V, V * w, atol=1e-11 * wr[0])
+
+
+def test_eigh_graded_and_degenerate(hip):
+    # graded spectrum down to 1e-14 relative and an exact 3-fold degeneracy
+    rng = np.random.default_rng(7)
+    Qm, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    lam = np.concatenate([[1, 1, 1], np.logspace(-1, -14, 37)])
+    G = (Qm * lam) @ Qm.T
+    w, V = hip.eigh(dev(G))
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    np.testing.assert_allclose(w, np.sort(lam)[::-1], atol=1e-15)
+    np.testing.assert_allclose(V.T @ V, np.eye(40), atol=1e-12)
+
+
+@pytest.mark.parametrize("m,q,rank", [(200, 20, 20), (1024, 66, 66), (300, 30, 1), (80, 22, 16),
+                                      (16, 16, 12)])
+def test_orthonormalize(hip, m, q, rank):
+    # q orthonormal columns always (rank-deficient sketches are completed, as
+    # Householder QR does), and their span contains range(Y)
+    rng = np.random.default_rng(m + q)
+    Y = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, q))
+    Qd = hip.orthonormalize(dev(Y)).cpu().numpy()
+    np.testing.assert_allclose(Qd.T @ Qd, np.eye(q), atol=1e-13)
+    np.testing.assert_allclose(Qd @ (Qd.T @ Y), Y, atol=1e-11 * np.abs(Y).max())
+
+
+@pytest.mark.parametrize("D,chi,k", [(2, 4, 0), (2, 8, 1), (3, 9, 2), (4, 16, 3), (3, 16, 0)])
+def test_quarter_tensor_matches_oracle(hip, D, chi, k):
+    A = oc.random_site_tensors(1, 1, D, 2, seed=D * 10 + k)[(0, 0)]
+    rng = np.random.default_rng(chi + k)
+    C = [rng.standard_normal((chi, chi)) for _ in range(4)]
+    E = [rng.standard_normal((chi, chi, D, D)) for _ in range(4)]
+    site = {'A': A, 'C': C, 'E': E}
+    Qo, _ = oc.make_quarter_tensor(site, k)
+    Qh = hip.quarter_tensor((dev(A), [dev(c) for c in C], [dev(e) for e in E]), k).cpu().numpy()
+    Qh = Qh / np.abs(Qh).max()
+    np.testing.assert_allclose(Qh, Qo, rtol=1e-12, atol=1e-13)

@@ -36,6 +36,9 @@ def _declare(lib):
     sig = {
         "acetn_hip_last_error": (ctypes.c_char_p, []),
         "acetn_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
+        "acetn_hip_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
+        "acetn_hip_kernel_timing_read": (ctypes.c_int, [ctypes.POINTER(c_dbl), ctypes.POINTER(c_dbl),
+                                                        ctypes.POINTER(c_i64)]),
         "acetn_hip_quarter_tensor_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int]),
         "acetn_hip_quarter_tensor": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int, c_vp,
                                                     c_vp, c_size, c_vp]),
@@ -47,11 +50,25 @@ def _declare(lib):
                                                      c_i64, ctypes.c_int, c_dbl, c_u64, c_u64, c_vp,
                                                      c_vp, ctypes.POINTER(c_i64), c_vp, c_vp, c_size,
                                                      c_vp]),
+        "acetn_hip_half_projectors_begin": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
+                                                           ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                           c_i64, c_i64, ctypes.c_int, c_dbl, c_u64,
+                                                           c_u64, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_half_projectors_end": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
+                                                         ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                         c_i64, c_i64, ctypes.c_int, c_i64, c_vp, c_vp,
+                                                         c_vp, c_size, c_vp]),
         "acetn_hip_renormalize_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                      c_i64, c_i64]),
         "acetn_hip_renormalize_boundary": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                           c_vp, c_vp, c_i64, c_vp, c_vp, c_i64,
                                                           c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_site_rdm_workspace": (c_size, [ctypes.POINTER(AcetnSite)]),
+        "acetn_hip_site_rdm": (ctypes.c_int, [ctypes.POINTER(AcetnSite), c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_bond_rdm_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.POINTER(AcetnSite),
+                                                  ctypes.c_int]),
+        "acetn_hip_bond_rdm": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.POINTER(AcetnSite),
+                                              ctypes.c_int, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_gaussian_sketch": (ctypes.c_int, [c_vp, c_i64, c_i64, c_u64, c_u64, c_vp]),
         "acetn_hip_gemm_workspace": (c_size, [c_i64, c_i64, c_i64]),
         "acetn_hip_gemm": (ctypes.c_int, [c_i64, c_i64, c_i64, ctypes.c_int, c_vp, c_i64,
@@ -129,10 +146,26 @@ class Workspace:
 
 
 _ws = Workspace()
+_slot_ws = {}
+_streams = {}
 
 
 def workspace(nbytes, device):
     return _ws.get(nbytes, device)
+
+
+def slot_workspace(slot, nbytes, device):
+    """Per-slot scratch for work that runs concurrently on several streams."""
+    w = _slot_ws.setdefault(slot, Workspace())
+    return w.get(nbytes, device)
+
+
+def slot_stream(slot, device):
+    """Default-priority side streams, one per concurrent slot."""
+    key = (slot, str(device))
+    if key not in _streams:
+        _streams[key] = torch.cuda.Stream(device=device)
+    return _streams[key]
 
 
 def make_site(A, C, E):
@@ -157,6 +190,17 @@ def device_info():
     buf = ctypes.create_string_buffer(512)
     check(lib().acetn_hip_device_info(buf, 512))
     return buf.value.decode()
+
+
+def kernel_timing(enable):
+    check(lib().acetn_hip_kernel_timing(int(enable)))
+
+
+def kernel_timing_read():
+    """(summed GEMM kernel ms, summed algorithmic flops, launches)."""
+    ms, fl, n = c_dbl(0), c_dbl(0), c_i64(0)
+    check(lib().acetn_hip_kernel_timing_read(ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(n)))
+    return ms.value, fl.value, n.value
 
 
 # ---------------------------------------------------------------- wrappers
@@ -237,8 +281,44 @@ def half_projectors(site1, site4, k, chi, oversampling, niter, cutoff, seed, str
     return P1[:rows * n].view(rows, n), P2[:rows * n].view(rows, n), spec
 
 
-def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j):
-    """Returns (C_a, C_b, E_o): the new C[(3+k)%4], C[k], E[(3+k)%4] of s2."""
+class ProjectorJob:
+    """One half-system projector pair computed in two phases (see
+    acetn_hip_half_projectors_begin/_end)."""
+
+    def __init__(self, slot, site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id):
+        self.s1, self.keep1 = make_site(*site1)
+        self.s4, self.keep4 = make_site(*site4)
+        self.k, self.chi, self.p, self.niter = k, chi, oversampling, niter
+        self.dev = self.keep1[0].device
+        self.rows = self.s1.E_dim[(3 + k) % 4][0] * self.s1.D * self.s1.D
+        self.slot = slot
+        self.stream = slot_stream(slot, self.dev)
+        nb = lib().acetn_hip_half_projectors_workspace(ctypes.byref(self.s1), ctypes.byref(self.s4),
+                                                       k, chi, oversampling)
+        self.ws = slot_workspace(slot, nb, self.dev)
+        self.cd_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
+        self.spec = torch.zeros(chi + oversampling, dtype=torch.float64, device=self.dev)
+        with torch.cuda.stream(self.stream):
+            check(lib().acetn_hip_half_projectors_begin(
+                ctypes.byref(self.s1), ctypes.byref(self.s4), k, chi, oversampling, niter, cutoff,
+                seed, stream_id, c_vp(self.cd_host.data_ptr()), ptr(self.spec), ptr(self.ws),
+                self.ws.numel(), c_vp(self.stream.cuda_stream)))
+
+    def finish(self):
+        """Call after the stream has been synchronised."""
+        n = int(self.cd_host[0])
+        with torch.cuda.stream(self.stream):
+            P1 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)
+            P2 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)
+            check(lib().acetn_hip_half_projectors_end(
+                ctypes.byref(self.s1), ctypes.byref(self.s4), self.k, self.chi, self.p, self.niter,
+                n, ptr(P1), ptr(P2), ptr(self.ws), self.ws.numel(), c_vp(self.stream.cuda_stream)))
+        return P1, P2
+
+
+def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None):
+    """Returns (C_a, C_b, E_o): the new C[(3+k)%4], C[k], E[(3+k)%4] of s2.
+    ``slot`` selects a per-slot workspace for concurrent calls."""
     s1, keep = make_site(*site1)
     dev = keep[0].device
     D = s1.D
@@ -250,8 +330,33 @@ def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j):
     Eo = torch.empty(cDi, cDj, D, D, dtype=torch.float64, device=dev)
     P1i, P2i, P1j, P2j = (t.contiguous() for t in (P1i, P2i, P1j, P2j))
     nb = lib().acetn_hip_renormalize_workspace(ctypes.byref(s1), k, cDi, cDj)
-    ws = workspace(nb, dev)
+    ws = workspace(nb, dev) if slot is None else slot_workspace(slot, nb, dev)
     check(lib().acetn_hip_renormalize_boundary(ctypes.byref(s1), k, ptr(P1i), ptr(P2i), cDi,
                                                ptr(P1j), ptr(P2j), cDj, ptr(Ca), ptr(Cb), ptr(Eo),
                                                ptr(ws), ws.numel(), stream_handle()))
     return Ca, Cb, Eo
+
+
+def site_rdm(site):
+    """rho[P][p] of one site; site = (A, [C0..C3], [E0..E3])."""
+    s, keep = make_site(*site)
+    dev = keep[0].device
+    rho = torch.empty(s.d, s.d, dtype=torch.float64, device=dev)
+    nb = lib().acetn_hip_site_rdm_workspace(ctypes.byref(s))
+    ws = workspace(nb, dev)
+    check(lib().acetn_hip_site_rdm(ctypes.byref(s), ptr(rho), ptr(ws), ws.numel(), stream_handle()))
+    return rho
+
+
+def bond_rdm(site1, site2, k):
+    """rho[P][Q][p][q] of the bond (s1, s2, k)."""
+    s1, keep1 = make_site(*site1)
+    s2, keep2 = make_site(*site2)
+    dev = keep1[0].device
+    d = s1.d
+    rho = torch.empty(d, d, d, d, dtype=torch.float64, device=dev)
+    nb = lib().acetn_hip_bond_rdm_workspace(ctypes.byref(s1), ctypes.byref(s2), k)
+    ws = workspace(nb, dev)
+    check(lib().acetn_hip_bond_rdm(ctypes.byref(s1), ctypes.byref(s2), k, ptr(rho), ptr(ws),
+                                   ws.numel(), stream_handle()))
+    return rho

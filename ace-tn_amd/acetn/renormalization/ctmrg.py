@@ -25,14 +25,13 @@ def _spectra_change(a, b):
 
 
 def ctmrg_iteration(mover, ipeps):
-    """One pass of ctmrg.py:26-31 (non-distributed order)."""
+    """One pass of ctmrg.py:26-31 (non-distributed order).  Each
+    left/right (up/down) pair is independent and runs as one stage."""
     nx, ny = ipeps.nx, ipeps.ny
     for xi in range(nx):
-        mover.left_move(ipeps, xi)
-        mover.right_move(ipeps, (nx - xi + 1) % nx)
+        mover.stage(ipeps, [(xi, 0), ((nx - xi + 1) % nx, 2)])
     for yi in range(ny):
-        mover.up_move(ipeps, (ny - yi + 1) % ny)
-        mover.down_move(ipeps, yi)
+        mover.stage(ipeps, [((ny - yi + 1) % ny, 1), (yi, 3)])
 
 
 def ctmrg(ipeps, config, mover=None):

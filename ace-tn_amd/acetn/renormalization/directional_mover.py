@@ -15,27 +15,51 @@ class DirectionalMover:
 
     # ---- moves: directional_mover.py:23-97
     def left_move(self, ipeps, xi):
-        self._move(ipeps, xi, 0)
+        self.stage(ipeps, [(xi, 0)])
 
     def up_move(self, ipeps, yi):
-        self._move(ipeps, yi, 1)
+        self.stage(ipeps, [(yi, 1)])
 
     def right_move(self, ipeps, xi):
-        self._move(ipeps, xi, 2)
+        self.stage(ipeps, [(xi, 2)])
 
     def down_move(self, ipeps, yi):
-        self._move(ipeps, yi, 3)
+        self.stage(ipeps, [(yi, 3)])
+
+    def stage(self, ipeps, moves):
+        """Run one or more moves whose projector computations and boundary
+        writes are mutually independent -- a left/right (or up/down) pair of
+        ctmrg.py:26-31 or fast_full_update.py:85-101: the left move of column
+        x reads C0,C3,E0,E2,E3 of column x and writes C0,C3,E3 of column x+1,
+        its right partner reads C1,C2,E0,E1,E2 and writes C1,C2,E1 -- so the
+        result equals the sequential order.  Sketch streams are numbered in
+        that sequential order."""
+        import torch
+        jobs, targets = [], []
+        for line, k in moves:
+            n_other = ipeps.ny if k in (0, 2) else ipeps.nx
+            base = len(jobs)
+            for idx in range(n_other):
+                xi, yi = (line, idx) if k in (0, 2) else (idx, line)
+                jobs.append((self.plaquette(ipeps.nx, ipeps.ny, xi, yi, k), k))
+            for idx in range(n_other):
+                s1, s2, i, j = self.targets(ipeps.nx, ipeps.ny, line, k, idx)
+                targets.append((s1, s2, base + i, base + j, k))
+        proj = self.projector_calculator.calculate_many(ipeps, jobs)
+        results = []
+        for slot, (s1, s2, i, j, k) in enumerate(targets):
+            stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
+            with torch.cuda.stream(stream):
+                results.append(self._renormalize(ipeps, proj[i], proj[j], s1, k, slot))
+        torch.cuda.synchronize()
+        for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(targets, results):
+            dst = ipeps[s2]
+            dst['C'][(3 + k) % 4] = Ca
+            dst['C'][k] = Cb
+            dst['E'][(3 + k) % 4] = Eo
 
     def _move(self, ipeps, line, k):
-        n_other = ipeps.ny if k in (0, 2) else ipeps.nx
-        proj1, proj2 = {}, {}
-        for idx in range(n_other):
-            xi, yi = (line, idx) if k in (0, 2) else (idx, line)
-            proj1[idx], proj2[idx] = self.calculate_projectors(
-                ipeps, self.plaquette(ipeps.nx, ipeps.ny, xi, yi, k), k)
-        for idx in range(n_other):
-            s1, s2, i, j = self.targets(ipeps.nx, ipeps.ny, line, k, idx)
-            self.renormalize_boundary(ipeps, proj1, proj2, s1, s2, i, j, k)
+        self.stage(ipeps, [(line, k)])
 
     # ---- calculate_{left,up,right,down}_projectors: directional_mover.py:99-181
     @staticmethod
@@ -78,12 +102,15 @@ class DirectionalMover:
     def renormalize_boundary(self, ipeps, proj1, proj2, s1, s2, i, j, k):
         """directional_mover.py:273-303: writes C[(3+k)%4], C[k], E[(3+k)%4]
         of s2."""
-        st = ipeps[s1]
-        flat = lambda P: P.reshape(-1, P.shape[-1])
-        Ca, Cb, Eo = _hip.renormalize_boundary((st['A'], st['C'], st['E']), k,
-                                               flat(proj1[i]), flat(proj2[i]),
-                                               flat(proj1[j]), flat(proj2[j]))
+        Ca, Cb, Eo = self._renormalize(ipeps, (proj1[i], proj2[i]), (proj1[j], proj2[j]), s1, k)
         dst = ipeps[s2]
         dst['C'][(3 + k) % 4] = Ca
         dst['C'][k] = Cb
         dst['E'][(3 + k) % 4] = Eo
+
+    @staticmethod
+    def _renormalize(ipeps, pi, pj, s1, k, slot=None):
+        st = ipeps[s1]
+        flat = lambda P: P.reshape(-1, P.shape[-1])
+        return _hip.renormalize_boundary((st['A'], st['C'], st['E']), k, flat(pi[0]), flat(pi[1]),
+                                         flat(pj[0]), flat(pj[1]), slot=slot)

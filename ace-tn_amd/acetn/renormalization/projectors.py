@@ -63,6 +63,37 @@ class ProjectorCalculator:
         rows = P1.shape[0]
         return P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)
 
+    def calculate_many(self, ipeps, jobs):
+        """Projector pairs of mutually independent plaquettes, ``jobs`` a list
+        of (sites, k) in the reference's call order (which fixes their sketch
+        streams).  Each pair runs on its own stream; one host synchronisation
+        reads all truncation ranks."""
+        if self.projectors != "half-system":
+            return [self.calculate(ipeps, sites, k) for sites, k in jobs]
+        if self.svd_type != "rsvd":
+            raise NotImplementedError("svd_type='full-rank' is not provided by the HIP engine; "
+                                      "use 'rsvd'")
+        import torch
+        chi, D = ipeps.dims['chi'], ipeps.dims['bond']
+        started = []
+        for slot, (sites, k) in enumerate(jobs):
+            sites = [tuple(s) for s in sites]
+            for s in sites:
+                ipeps[s]
+            started.append(_hip.ProjectorJob(slot, self._site(ipeps, sites[0]),
+                                             self._site(ipeps, sites[3]), k % 4, chi,
+                                             self.rsvd_oversampling, self.rsvd_niter,
+                                             self.svd_cutoff, self.sketch.seed, self.sketch.take()))
+        torch.cuda.synchronize()
+        out = []
+        for job in started:
+            P1, P2 = job.finish()
+            rows = P1.shape[0]
+            out.append((P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)))
+        self.last_spectrum = started[-1].spec
+        torch.cuda.synchronize()
+        return out
+
     def calculate_full_system(self, ipeps, sites, k):
         raise NotImplementedError("projectors='full-system' is not provided by the HIP engine yet; "
                                   "use 'half-system'")

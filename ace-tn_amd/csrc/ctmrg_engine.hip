@@ -21,7 +21,8 @@ thread_local std::string g_last_error;
 struct Ctx {
   char* base = nullptr;
   size_t cap = 0, off = 0;
-  bool run = false;
+  bool run = false;     // buffers are carved out of the workspace
+  bool launch = false;  // kernels are enqueued (false while replaying an earlier phase)
   hipStream_t st = nullptr;
   double* gwork = nullptr;
   size_t gwork_elems = 0;
@@ -53,6 +54,7 @@ void execute(void* ws, size_t ws_bytes, void* stream, F&& body) {
   body(m);  // sizes
   Ctx c;
   c.run = true;
+  c.launch = true;
   c.st = reinterpret_cast<hipStream_t>(stream);
   uintptr_t b = reinterpret_cast<uintptr_t>(ws);
   uintptr_t aligned = (b + 255) & ~uintptr_t(255);
@@ -71,6 +73,7 @@ void gemm(Ctx& c, const GemmDesc& g, const double* A, const double* B, double* C
     c.gwork_need = std::max(c.gwork_need, gemm_f64_work_elems(g, 1));
     return;
   }
+  if (!c.launch) return;
   GemmBatch b{};
   b.count = 1;
   b.A[0] = A; b.B[0] = B; b.C[0] = C;
@@ -144,7 +147,7 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
   g.cm = plain(qd.N2); g.cn = plain(1);
   g.alpha = 1.0; g.beta = 0.0;
   gemm(c, g, T1, s->E[(3 + kk) & 3], T2);
-  if (!c.run) return;
+  if (!c.launch) return;
   int64_t ps[4];
   permuted_strides(s, kk, ps);
   DLDesc dl{};
@@ -181,10 +184,10 @@ void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
     double* dst = (pass % 2 == 0) ? Yt : Y;
     mm(c, (int)q, (int)q, (int)m, true, src, q, false, src, q, G, q);
     const double shift = (pass == 0) ? 11.0 * ((double)m * q + (double)q * (q + 1)) * u : 0.0;
-    if (c.run) chol_rinv(G, Rinv, (int)q, shift, pass == 0 ? 0.0 : 1e-14, c.st, scr,
-                         pass == 1 ? dead : nullptr);
+    if (c.launch) chol_rinv(G, Rinv, (int)q, shift, pass == 0 ? 0.0 : 1e-14, c.st, scr,
+                         pass == 1 ? dead : nullptr, pass >= 2 ? dead : nullptr);
     mm(c, (int)m, (int)q, (int)q, false, src, q, false, Rinv, q, dst, q);
-    if (pass == 1 && c.run) refill_dead_columns(dst, m, (int)q, dead, 0x243F6A8885A308D3ull, c.st);
+    if (pass == 1 && c.launch) refill_dead_columns(dst, m, (int)q, dead, 0x243F6A8885A308D3ull, c.st);
   }
   // four passes leave the result in Y
 }
@@ -203,10 +206,11 @@ void small_svd(Ctx& c, const double* BT, int64_t n, int64_t q, double* w, double
   double* B2T = c.take((size_t)n * q);
   double* rot = c.take(jacobi_rot_elems((int)q));
   mm(c, (int)q, (int)q, (int)n, true, BT, q, false, BT, q, G, q);
-  if (c.run) jacobi_eigh(G, w1, W1, rot, (int)q, c.st);
+  // pass 1 only preconditions (pass 2 converges quadratically from ~1e-9)
+  if (c.launch) jacobi_eigh(G, w1, W1, rot, (int)q, c.st, 1e-9);
   mm(c, (int)n, (int)q, (int)q, false, BT, q, false, W1, q, B2T, q);
   mm(c, (int)q, (int)q, (int)n, true, B2T, q, false, B2T, q, G, q);
-  if (c.run) jacobi_eigh(G, w, W2, rot, (int)q, c.st);
+  if (c.launch) jacobi_eigh(G, w, W2, rot, (int)q, c.st);
   mm(c, (int)q, (int)q, (int)q, false, W1, q, false, W2, q, UB, q);
 }
 
@@ -217,13 +221,18 @@ void small_svd(Ctx& c, const double* BT, int64_t n, int64_t q, double* w, double
 struct HalfOut {
   double* P1;
   double* P2;
-  int64_t* host_cD;
+  int64_t* host_cD;   // phase 0: written after a stream sync
+  int* pinned_cD;     // phase 1: async copy target (pinned host memory)
+  int cD_in;          // phase 2: cD_new read by the host after phase 1
   double* spectrum;
 };
 
+// phase 0: everything with one host sync; phase 1: through the truncation
+// count (async copy of cD_new to pinned host memory); phase 2: the projector
+// products, given cD_new.  Phases 1 and 2 carve identical workspace layouts.
 void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
                      int64_t oversampling, int niter, double cutoff, uint64_t seed,
-                     uint64_t sstream, const HalfOut& out) {
+                     uint64_t sstream, const HalfOut& out, int phase = 0) {
   const QuarterDims d1 = quarter_dims(s1, k);
   const QuarterDims d4 = quarter_dims(s4, k + 3);
   ACETN_REQUIRE(d1.N2 == d4.N1, "half system: Q1 columns != Q4 rows");
@@ -245,9 +254,11 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   double* X = c.take((size_t)q * q);
   double* Vp = c.take((size_t)n * q);
 
+  const bool replay = c.run && phase == 2;
+  if (replay) c.launch = false;
   quarter_tensor(c, s1, k, Q1);
   quarter_tensor(c, s4, k + 3, Q4);
-  if (c.run) gaussian_sketch(Om, n, q, seed, sstream, c.st);
+  if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st);
   // Y = Q1 (Q4 Omega)
   mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Om, q, W, q);
   mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);
@@ -271,13 +282,22 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
     mm(c, (int)kk, cDp, (int)n, false, nullptr, n, false, nullptr, cDp, nullptr, cDp);
     return;
   }
-  count_kept(w, (int)q, (int)chi, cutoff, s, cD_dev, c.st);
   int cD = 0;
-  ACETN_HIP_CHECK(hipMemcpyAsync(&cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
-  ACETN_HIP_CHECK(hipStreamSynchronize(c.st));
-  ACETN_REQUIRE(cD >= 1, "half system: no singular value above the cutoff");
-  *out.host_cD = cD;
-  if (out.spectrum) ACETN_HIP_CHECK(hipMemcpyAsync(out.spectrum, s, sizeof(double) * q, hipMemcpyDeviceToDevice, c.st));
+  if (phase == 2) {
+    c.launch = true;
+    cD = out.cD_in;
+  } else {
+    count_kept(w, (int)q, (int)chi, cutoff, s, cD_dev, c.st);
+    if (out.spectrum) ACETN_HIP_CHECK(hipMemcpyAsync(out.spectrum, s, sizeof(double) * q, hipMemcpyDeviceToDevice, c.st));
+    if (phase == 1) {
+      ACETN_HIP_CHECK(hipMemcpyAsync(out.pinned_cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+      return;
+    }
+    ACETN_HIP_CHECK(hipMemcpyAsync(&cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+    ACETN_HIP_CHECK(hipStreamSynchronize(c.st));
+    *out.host_cD = cD;
+  }
+  ACETN_REQUIRE(cD >= 1 && cD <= std::min<int64_t>(chi, q), "half system: no singular value above the cutoff");
   // P1 = Q1^H U S_n^-1/2 = Wq (UB S_n^-1/2)
   scale_columns(UB, (int)q, s, cD, 0, X, c.st);
   mm(c, (int)kk, cD, (int)q, false, Wq, q, false, X, cD, out.P1, cD);
@@ -318,7 +338,7 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     h.cm = plain(cDi); h.cn = plain(1);
     h.alpha = 1.0; h.beta = 0.0;
     gemm(c, h, T, P1i, Ca);
-    if (c.run) frobenius_normalize(Ca, xa * cDi, nrm, c.st);
+    if (c.launch) frobenius_normalize(Ca, xa * cDi, nrm, c.st);
   }
   // ---- cj2: C[k] with E[k] and P2_j
   {
@@ -335,7 +355,7 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     h.cm = plain(xc); h.cn = plain(1);
     h.alpha = 1.0; h.beta = 0.0;
     gemm(c, h, P2j, T, Cb);
-    if (c.run) frobenius_normalize(Cb, cDj * xc, nrm, c.st);
+    if (c.launch) frobenius_normalize(Cb, cDj * xc, nrm, c.st);
   }
   // ---- ej: E[(3+k)] with A_k, P1_j, P2_i
   {
@@ -350,7 +370,7 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     g.cm = plain(D2 * cDj); g.cn = plain(1);
     g.alpha = 1.0; g.beta = 0.0;
     gemm(c, g, s1->E[ke], P1j, T);
-    if (c.run) {
+    if (c.launch) {
       int64_t ps[4];
       permuted_strides(s1, k, ps);
       DLDesc dl{};
@@ -373,8 +393,180 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     h.cm = plain(1); h.cn = plain(cDj * D2);
     h.alpha = 1.0; h.beta = 0.0;
     gemm(c, h, Zt, P2i, Eo);
-    if (c.run) frobenius_normalize(Eo, cDi * cDj * D2, nrm, c.st);
+    if (c.launch) frobenius_normalize(Eo, cDi * cDj * D2, nrm, c.st);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Reduced density matrices (acetn/measurement/rdm.py)
+// ---------------------------------------------------------------------------
+GemmDesc gdesc(int M, int N, int K, IdxMap am, IdxMap ak, IdxMap bk, IdxMap bn, IdxMap cm,
+               IdxMap cn) {
+  GemmDesc g{};
+  g.M = M; g.N = N; g.K = K;
+  g.am = am; g.ak = ak; g.bk = bk; g.bn = bn; g.cm = cm; g.cn = cn;
+  g.alpha = 1.0; g.beta = 0.0;
+  return g;
+}
+
+// build_site_rdm, rdm.py:35-67.  rho (d x d) = "Pp".
+void site_rdm(Ctx& c, const acetn_site* s, double* rho) {
+  const int64_t D = s->D, d = s->d, D2 = D * D;
+  const int64_t a0 = s->C_dim[3][0], b0 = s->C_dim[3][1];           // c4 (a,b)
+  ACETN_REQUIRE(s->E_dim[3][0] == b0, "site rdm: c4/e4 mismatch");
+  const int64_t c4 = s->E_dim[3][1];                                // e4 (b,c,l,L)
+  ACETN_REQUIRE(s->E_dim[2][1] == a0, "site rdm: c4/e3 mismatch");
+  const int64_t e3 = s->E_dim[2][0];                                // e3 (e,a,d,D)
+  // t1a[a][c][l][L] = c4 . e4
+  double* t1a = c.take((size_t)a0 * c4 * D2);
+  mm(c, (int)a0, (int)(c4 * D2), (int)b0, false, s->C[3], b0, false, s->E[3], c4 * D2, t1a, c4 * D2);
+  // t1b[c][l][L][e][d][D] = sum_a t1a[a][(c,l,L)] e3[e][a][(d,D)]
+  const int64_t N1 = c4 * D2, Ne = e3 * D2;
+  double* t1b = c.take((size_t)N1 * Ne);
+  gemm(c, gdesc((int)N1, (int)Ne, (int)a0, plain(1), plain(N1), plain(D2),
+                grouped((int)D2, 1, a0 * D2), plain(Ne), plain(1)), t1a, s->E[2], t1b);
+  // t1c[c][l][e][d][U][R][P] = sum_{L,D} t1b[c][l][L][e][d][D] abar[L][U][R][D][P]
+  const int64_t Mc = c4 * D * e3 * D, Nc = D2 * d;
+  double* t1c = c.take((size_t)Mc * Nc);
+  gemm(c, gdesc((int)Mc, (int)Nc, (int)D2,
+                grouped4((int)D, D, (int)e3, D2, (int)D, D * Ne, D2 * Ne),  // m = (c,l,e,d)
+                grouped((int)D, 1, Ne),                                   // k = (L,D)
+                grouped((int)D, d, D * D2 * d),                           // A: D -> d, L -> D^3 d
+                grouped3((int)d, 1, (int)D, D * d, D2 * d),               // n = (U,R,P)
+                plain(Nc), plain(1)), t1b, s->A, t1c);
+  // t2[a][c][u][U] = c1 . e1
+  const int64_t a1 = s->C_dim[0][0], b1 = s->C_dim[0][1];
+  ACETN_REQUIRE(s->E_dim[0][0] == b1, "site rdm: c1/e1 mismatch");
+  const int64_t c1 = s->E_dim[0][1];
+  double* t2 = c.take((size_t)a1 * c1 * D2);
+  mm(c, (int)a1, (int)(c1 * D2), (int)b1, false, s->C[0], b1, false, s->E[0], c1 * D2, t2, c1 * D2);
+  // t3a[b][c][r][R] = sum_a c3[a][b] e2[c][a][r][R]
+  const int64_t a3 = s->C_dim[2][0], b3 = s->C_dim[2][1];
+  ACETN_REQUIRE(s->E_dim[1][1] == a3, "site rdm: c3/e2 mismatch");
+  const int64_t c2 = s->E_dim[1][0];
+  double* t3a = c.take((size_t)b3 * c2 * D2);
+  gemm(c, gdesc((int)b3, (int)(c2 * D2), (int)a3, plain(1), plain(b3), plain(D2),
+                grouped((int)D2, 1, a3 * D2), plain(c2 * D2), plain(1)), s->C[2], s->E[1], t3a);
+  // t3b[e][b][r][R] = sum_c c2[e][c] t3a[b][c][r][R]
+  const int64_t e2 = s->C_dim[1][0];
+  ACETN_REQUIRE(s->C_dim[1][1] == c2, "site rdm: c2/e2 mismatch");
+  double* t3b = c.take((size_t)e2 * b3 * D2);
+  gemm(c, gdesc((int)e2, (int)(b3 * D2), (int)c2, plain(c2), plain(1), plain(D2),
+                grouped((int)D2, 1, c2 * D2), plain(b3 * D2), plain(1)), s->C[1], t3a, t3b);
+  // t3c[b][r][R][a][u][U] = sum_e t3b[e][(b,r,R)] t2[a][e][(u,U)]
+  ACETN_REQUIRE(c1 == e2, "site rdm: e1/c2 mismatch");
+  const int64_t Mb = b3 * D2, Na = a1 * D2;
+  double* t3c = c.take((size_t)Mb * Na);
+  gemm(c, gdesc((int)Mb, (int)Na, (int)e2, plain(1), plain(Mb), plain(D2),
+                grouped((int)D2, 1, c1 * D2), plain(Na), plain(1)), t3b, t2, t3c);
+  // t4[r][u][l][d][P] = sum_{e,R,c,U} t3c[e][r][R][c][u][U] t1c[c][l][e][d][U][R][P]
+  //   (t3c's leading index b plays e, its a plays c)
+  ACETN_REQUIRE(b3 == e3 && a1 == c4, "site rdm: ring dimension mismatch");
+  const int64_t ne = b3, nc = a1, Kt = ne * D * nc * D;
+  double* t4 = c.take((size_t)D2 * D2 * d);
+  gemm(c, gdesc((int)D2, (int)(D2 * d), (int)Kt,
+                grouped((int)D, D, D * nc * D2),                               // m = (r,u)
+                grouped4((int)D, 1, (int)nc, D2, (int)D, nc * D2, D2 * nc * D2),  // k = (e,R,c,U)
+                grouped4((int)D, D * d, (int)nc, D * e3 * D * D2 * d, (int)D, d, D * D2 * d),
+                grouped3((int)d, 1, (int)D, D2 * d, e3 * D * D2 * d),           // n = (l,d,P)
+                plain(D2 * d), plain(1)), t3c, t1c, t4);
+  // rho[P][p] = sum_{r,u,l,d} t4[(r,u,l,d)][P] a[l][u][r][d][p]
+  gemm(c, gdesc((int)d, (int)d, (int)(D2 * D2), plain(1), plain(d),
+                grouped4((int)D, d, (int)D, D * D2 * d, (int)D, D2 * d, D * d),
+                plain(1), plain(d), plain(1)), t4, s->A, rho);
+}
+
+// build_bond_rdm + build_bond_rdm_core, rdm.py:69-120 (the blocked form of
+// :123-154 computes the same tensor).  rho (d,d,d,d) = "PQpq".
+void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double* rho) {
+  k &= 3;
+  const int64_t D = s1->D, d = s1->d, D2 = D * D, d2 = d * d;
+  ACETN_REQUIRE(s2->D == D && s2->d == d, "bond rdm: site dimensions differ");
+  const int k1 = (k + 1) & 3, k2 = (k + 2) & 3, k3 = (k + 3) & 3;
+  int64_t ps1[4], ps2[4];
+  permuted_strides(s1, k, ps1);
+  permuted_strides(s2, k, ps2);
+  // ---------------- right half (site s1)
+  const int64_t c12a = s1->C_dim[k1][0], c12b = s1->C_dim[k1][1];
+  ACETN_REQUIRE(s1->E_dim[k1][0] == c12b, "bond rdm: c12/e12 mismatch");
+  const int64_t nc = s1->E_dim[k1][1];                 // c range
+  ACETN_REQUIRE(s1->E_dim[k][1] == c12a, "bond rdm: c12/e11 mismatch");
+  const int64_t ne = s1->E_dim[k][0];                  // e range
+  const int64_t Mr = nc * D2, Nr = ne * D2;
+  double* rt1 = c.take((size_t)c12a * Mr);
+  mm(c, (int)c12a, (int)Mr, (int)c12b, false, s1->C[k1], c12b, false, s1->E[k1], Mr, rt1, Mr);
+  double* rt2 = c.take((size_t)Mr * Nr);
+  gemm(c, gdesc((int)Mr, (int)Nr, (int)c12a, plain(1), plain(Mr), plain(D2),
+                grouped((int)D2, 1, c12a * D2), plain(Nr), plain(1)), rt1, s1->E[k], rt2);
+  const int64_t Lr = ne * D2 * d2;                     // row length of t: (c,L,P,l,p)
+  double* t = c.take((size_t)nc * D2 * Lr);
+  if (c.launch) {
+    DLDesc dl{};
+    dl.D = (int)D; dl.d = (int)d; dl.n1 = (int)nc; dl.n2 = (int)ne;
+    dl.x_o1 = D2 * Nr; dl.x_k2 = D * Nr; dl.x_b2 = Nr;   // (c, r, R)
+    dl.x_o2 = D2; dl.x_k1 = D; dl.x_b1 = 1;              // (e, u, U)
+    dl.a_leg[0] = ps1[1]; dl.a_leg[1] = ps1[2]; dl.a_leg[2] = ps1[0]; dl.a_leg[3] = ps1[3];
+    dl.a_p = 1;
+    dl.open_p = 1;
+    // t[a=c][d][D][c=e][L][P][l][p]
+    dl.y_o1 = D2 * Lr; dl.y_k4 = D * Lr; dl.y_b4 = Lr;
+    dl.y_o2 = D2 * d2; dl.y_b3 = D * d2; dl.y_pb = D * d; dl.y_k3 = d; dl.y_p = 1;
+    double_layer_f64(dl, rt2, s1->A, t, c.st);
+  }
+  const int64_t c13a = s1->C_dim[k2][0], c13b = s1->C_dim[k2][1];
+  ACETN_REQUIRE(s1->E_dim[k2][0] == c13b, "bond rdm: c13/e13 mismatch");
+  ACETN_REQUIRE(c13a == nc, "bond rdm: c13/e12 mismatch");
+  const int64_t F = s1->E_dim[k2][1];
+  double* r1a = c.take((size_t)c13a * F * D2);
+  mm(c, (int)c13a, (int)(F * D2), (int)c13b, false, s1->C[k2], c13b, false, s1->E[k2], F * D2, r1a,
+     F * D2);
+  double* r1 = c.take((size_t)F * Lr);
+  gemm(c, gdesc((int)F, (int)Lr, (int)(nc * D2), plain(D2), grouped((int)D2, 1, F * D2), plain(Lr),
+                plain(1), plain(Lr), plain(1)), r1a, t, r1);
+  // ---------------- left half (site s2)
+  const int64_t c21a = s2->C_dim[k][0], c21b = s2->C_dim[k][1];
+  ACETN_REQUIRE(s2->E_dim[k][0] == c21b, "bond rdm: c21/e21 mismatch");
+  const int64_t mc = s2->E_dim[k][1];
+  ACETN_REQUIRE(s2->E_dim[k3][1] == c21a, "bond rdm: c21/e24 mismatch");
+  const int64_t me = s2->E_dim[k3][0];
+  const int64_t Ml = mc * D2, Nl = me * D2;
+  double* lt1 = c.take((size_t)c21a * Ml);
+  mm(c, (int)c21a, (int)Ml, (int)c21b, false, s2->C[k], c21b, false, s2->E[k], Ml, lt1, Ml);
+  double* lt2 = c.take((size_t)Ml * Nl);
+  gemm(c, gdesc((int)Ml, (int)Nl, (int)c21a, plain(1), plain(Ml), plain(D2),
+                grouped((int)D2, 1, c21a * D2), plain(Nl), plain(1)), lt1, s2->E[k3], lt2);
+  const int64_t Ll = mc * D2 * d2;                     // row length of t2: (c,R,Q,r,q)
+  double* t2 = c.take((size_t)me * D2 * Ll);
+  if (c.launch) {
+    DLDesc dl{};
+    dl.D = (int)D; dl.d = (int)d; dl.n1 = (int)mc; dl.n2 = (int)me;
+    dl.x_o1 = D2 * Nl; dl.x_k1 = D * Nl; dl.x_b1 = Nl;   // (c, u, U)
+    dl.x_o2 = D2; dl.x_k2 = D; dl.x_b2 = 1;              // (e, l, L)
+    dl.a_leg[0] = ps2[1]; dl.a_leg[1] = ps2[0]; dl.a_leg[2] = ps2[2]; dl.a_leg[3] = ps2[3];
+    dl.a_p = 1;
+    dl.open_p = 1;
+    // t2[e][d][D][c][R][Q][r][q]
+    dl.y_o2 = D2 * Ll; dl.y_k4 = D * Ll; dl.y_b4 = Ll;
+    dl.y_o1 = D2 * d2; dl.y_b3 = D * d2; dl.y_pb = D * d; dl.y_k3 = d; dl.y_p = 1;
+    double_layer_f64(dl, lt2, s2->A, t2, c.st);
+  }
+  const int64_t c24a = s2->C_dim[k3][0], c24b = s2->C_dim[k3][1];
+  ACETN_REQUIRE(c24b == me, "bond rdm: c24/e24 mismatch");
+  ACETN_REQUIRE(s2->E_dim[k2][1] == c24a, "bond rdm: c24/e23 mismatch");
+  const int64_t F2 = s2->E_dim[k2][0];
+  double* r2a = c.take((size_t)me * F2 * D2);
+  gemm(c, gdesc((int)me, (int)(F2 * D2), (int)c24a, plain(1), plain(c24b), plain(D2),
+                grouped((int)D2, 1, c24a * D2), plain(F2 * D2), plain(1)), s2->C[k3], s2->E[k2], r2a);
+  double* r2 = c.take((size_t)F2 * Ll);
+  gemm(c, gdesc((int)F2, (int)Ll, (int)(me * D2), plain(D2), grouped((int)D2, 1, F2 * D2),
+                plain(Ll), plain(1), plain(Ll), plain(1)), r2a, t2, r2);
+  // ---------------- rho[P][Q][p][q] = sum_{f,c,R,r} r1[f][c][R][P][r][p] r2[f][c][R][Q][r][q]
+  ACETN_REQUIRE(F == F2 && ne == mc, "bond rdm: halves do not close");
+  const int64_t K = F * ne * D2;
+  gemm(c, gdesc((int)d2, (int)d2, (int)K, grouped((int)d, 1, D * d),
+                grouped3((int)D, d, (int)D, D * d2, D2 * d2), grouped3((int)D, d, (int)D, D * d2, D2 * d2),
+                grouped((int)d, 1, D * d), grouped((int)d, d, d2 * d), grouped((int)d, 1, d2)),
+       r1, r2, rho);
 }
 
 template <class F>
@@ -402,6 +594,14 @@ using namespace acetn;
 extern "C" {
 
 const char* acetn_hip_last_error(void) { return g_last_error.c_str(); }
+
+int acetn_hip_kernel_timing(int enable) {
+  return guarded([&] { kernel_timing_enable(enable != 0); });
+}
+
+int acetn_hip_kernel_timing_read(double* host_ms, double* host_flops, int64_t* host_launches) {
+  return guarded([&] { kernel_timing_read(host_ms, host_flops, host_launches); });
+}
 
 int acetn_hip_device_info(char* buf, size_t len) {
   return guarded([&] {
@@ -434,7 +634,7 @@ size_t acetn_hip_half_projectors_workspace(const acetn_site* s1, const acetn_sit
   guarded([&] {
     check_site(s1);
     check_site(s4);
-    HalfOut o{nullptr, nullptr, nullptr, nullptr};
+    HalfOut o{nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     r = measure([&](Ctx& c) {
       half_projectors(c, s1, s4, k, chi, oversampling, 2, 0.0, 0, 0, o);
     });
@@ -452,9 +652,39 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
     check_site(s4);
     ACETN_REQUIRE(P1 && P2 && host_cD_new, "half_projectors: null output");
     ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
-    HalfOut o{P1, P2, host_cD_new, spectrum};
+    HalfOut o{P1, P2, host_cD_new, nullptr, 0, spectrum};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o);
+    });
+  });
+}
+
+int acetn_hip_half_projectors_begin(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
+                                    int64_t oversampling, int niter, double svd_cutoff,
+                                    uint64_t seed, uint64_t sketch_stream, int32_t* pinned_cD_new,
+                                    double* spectrum, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_site(s1);
+    check_site(s4);
+    ACETN_REQUIRE(pinned_cD_new, "half_projectors_begin: null cD target");
+    ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
+    HalfOut o{nullptr, nullptr, nullptr, pinned_cD_new, 0, spectrum};
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o, 1);
+    });
+  });
+}
+
+int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
+                                  int64_t oversampling, int niter, int64_t cD_new, double* P1,
+                                  double* P2, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_site(s1);
+    check_site(s4);
+    ACETN_REQUIRE(P1 && P2, "half_projectors_end: null output");
+    HalfOut o{P1, P2, nullptr, nullptr, (int)cD_new, nullptr};
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      half_projectors(c, s1, s4, k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
     });
   });
 }
@@ -482,6 +712,40 @@ int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o);
     });
+  });
+}
+
+size_t acetn_hip_site_rdm_workspace(const acetn_site* s) {
+  size_t r = 0;
+  guarded([&] { check_site(s); r = measure([&](Ctx& c) { site_rdm(c, s, nullptr); }); });
+  return r;
+}
+
+int acetn_hip_site_rdm(const acetn_site* s, double* rho, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_site(s);
+    ACETN_REQUIRE(rho, "site_rdm: null output");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) { site_rdm(c, s, rho); });
+  });
+}
+
+size_t acetn_hip_bond_rdm_workspace(const acetn_site* s1, const acetn_site* s2, int k) {
+  size_t r = 0;
+  guarded([&] {
+    check_site(s1);
+    check_site(s2);
+    r = measure([&](Ctx& c) { bond_rdm(c, s1, s2, k, nullptr); });
+  });
+  return r;
+}
+
+int acetn_hip_bond_rdm(const acetn_site* s1, const acetn_site* s2, int k, double* rho, void* ws,
+                       size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_site(s1);
+    check_site(s2);
+    ACETN_REQUIRE(rho, "bond_rdm: null output");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) { bond_rdm(c, s1, s2, k, rho); });
   });
 }
 

@@ -7,101 +7,192 @@
 // This is the pair of einsums "cuUelL,LURDP->cuelRDP" + "lurdp,cuelRDp->crRedD"
 // of make_quarter_tensor (acetn/renormalization/projectors.py:54-55) and the
 // pair "LURDP,alLuUx->RDPalux" + "lurdp,RDpalux->rdRDax" of renormalize_ej
-// (acetn/renormalization/directional_mover.py:363-364): for a fixed boundary
-// pair (o1,o2) the work is two (D^2 x D^2 x D^2) products per physical index,
-// done in LDS, so the 7-index chi^2 D^4 d intermediate never reaches HBM.
-// A is re-read from LDS for every block (it is loaded once per workgroup);
-// X and Y are read/written exactly once.
+// (acetn/renormalization/directional_mover.py:363-364).
+//
+// For a fixed boundary pair the work is two (D^2 x D^2 x D^2) products per
+// physical index p:  T = X . Abar_p  and  Y += A_p^T . T.  NB blocks are
+// stacked so that the row count is a multiple of the 16-row MFMA tile
+// (NB = 4 at D = 6 gives exactly 144 rows), and both products run on
+// v_mfma_f64_16x16x4_f64 from LDS.  Y stays in registers across p; the
+// 7-index chi^2 D^4 d intermediate never reaches HBM, X and Y are read and
+// written once.
 #include "internal.h"
 
 namespace acetn {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
 namespace {
 constexpr int NT = 256;
-constexpr int MAXPER = 16;  // D^4 / NT for D <= 8
 
-__global__ __launch_bounds__(NT) void double_layer_kernel(DLDesc g, const double* __restrict__ X,
-                                                          const double* __restrict__ A,
-                                                          double* __restrict__ Y) {
+template <int D>
+struct DLShape {
+  static constexpr int D2 = D * D;
+  static constexpr int NB = (D == 6) ? 4 : ((64 / D2) > 0 ? (64 / D2) : 1);
+  static constexpr int ROWS = NB * D2;            // stacked rows
+  static constexpr int MP = (ROWS + 15) / 16 * 16; // padded rows (step c M, step d N)
+  static constexpr int KP = (D2 + 3) / 4 * 4;      // padded contraction length
+  static constexpr int NP = (D2 + 15) / 16 * 16;   // padded (b3,b4) columns / step d rows
+  static constexpr int TC = (MP / 16) * (NP / 16); // tiles per product
+  static constexpr int TPW = (TC + 3) / 4;         // tiles per wave
+};
+
+template <int D>
+__global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __restrict__ X,
+                                                     const double* __restrict__ A,
+                                                     double* __restrict__ Y) {
+  using S = DLShape<D>;
   extern __shared__ double sm[];
-  const int D = g.D, d = g.d, D2 = D * D, D4 = D2 * D2;
-  double* Ac = sm;           // [l1][l2][l3][l4][p]
-  double* Xs = Ac + D4 * d;  // [(k1,k2)][(b1,b2)]
-  double* Ts = Xs + D4;      // [(k1,k2)][(b3,b4)] for one p
-  const int tid = threadIdx.x;
+  const int d = g.d;
+  double* Ac = sm;                          // [p][KP][NP]: A[(l1,l2),(l3,l4),p], zero padded
+  double* Xs = Ac + (size_t)d * S::KP * S::NP;  // [MP][KP]: stacked blocks, rows (blk,k1,k2)
+  double* Ts = Xs + S::MP * S::KP;              // [MP][NP]: rows (blk,k1,k2), cols (b3,b4)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  for (int e = tid; e < D4 * d; e += NT) {
-    int r = e;
-    const int p = r % d; r /= d;
-    const int l4 = r % D; r /= D;
-    const int l3 = r % D; r /= D;
-    const int l2 = r % D;
-    const int l1 = r / D;
-    Ac[e] = A[l1 * g.a_leg[0] + l2 * g.a_leg[1] + l3 * g.a_leg[2] + l4 * g.a_leg[3] + p * g.a_p];
+  for (int e = tid; e < d * S::KP * S::NP; e += NT) {
+    const int p = e / (S::KP * S::NP), r = e % (S::KP * S::NP);
+    const int kk = r / S::NP, nn = r % S::NP;
+    double v = 0.0;
+    if (kk < S::D2 && nn < S::D2) {
+      const int l1 = kk / D, l2 = kk % D, l3 = nn / D, l4 = nn % D;
+      v = A[l1 * g.a_leg[0] + l2 * g.a_leg[1] + l3 * g.a_leg[2] + l4 * g.a_leg[3] + p * g.a_p];
+    }
+    Ac[e] = v;
   }
 
   const int nblk = g.n1 * g.n2;
-  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-    const int o1 = blk / g.n2, o2 = blk % g.n2;
-    const int64_t xb = (int64_t)o1 * g.x_o1 + (int64_t)o2 * g.x_o2;
+  const int ngroups = (nblk + S::NB - 1) / S::NB;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int b0 = grp * S::NB;
     __syncthreads();
-    for (int e = tid; e < D4; e += NT) {
-      const int row = e / D2, col = e % D2;
-      const int k1 = row / D, k2 = row % D, b1 = col / D, b2 = col % D;
-      Xs[e] = X[xb + k1 * g.x_k1 + b1 * g.x_b1 + k2 * g.x_k2 + b2 * g.x_b2];
+    // ---- stacked X tile
+    for (int e = tid; e < S::MP * S::KP; e += NT) {
+      const int row = e / S::KP, col = e % S::KP;
+      double v = 0.0;
+      const int blk = row / S::D2, kk = row % S::D2;
+      if (row < S::ROWS && col < S::D2 && b0 + blk < nblk) {
+        const int o1 = (b0 + blk) / g.n2, o2 = (b0 + blk) % g.n2;
+        const int k1 = kk / D, k2 = kk % D, b1 = col / D, b2 = col % D;
+        v = X[(int64_t)o1 * g.x_o1 + (int64_t)o2 * g.x_o2 + k1 * g.x_k1 + b1 * g.x_b1 +
+              k2 * g.x_k2 + b2 * g.x_b2];
+      }
+      Xs[e] = v;
     }
-    double yacc[MAXPER];
+    d4 yacc[S::TPW];
 #pragma unroll
-    for (int i = 0; i < MAXPER; ++i) yacc[i] = 0.0;
+    for (int t = 0; t < S::TPW; ++t) yacc[t] = d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int p = 0; p < d; ++p) {
+    // closed: sum_p A_p^T (X Abar_p);  open: every (p, pb) pair written out
+    for (int pb = 0; pb < d; ++pb) {
+      const double* Apb = Ac + (size_t)pb * S::KP * S::NP;
       __syncthreads();
-      // Ts[(k1,k2),(b3,b4)] = sum_{(b1,b2)} Xs[(k1,k2),(b1,b2)] * A[(b1,b2),(b3,b4),p]
-      for (int e = tid; e < D4; e += NT) {
-        const int row = e / D2, col = e % D2;
-        const double* xr = Xs + row * D2;
-        double s = 0.0;
-        for (int kk = 0; kk < D2; ++kk) s += xr[kk] * Ac[(kk * D2 + col) * d + p];
-        Ts[e] = s;
+      // ---- step c: Ts[MP x NP] = Xs[MP x KP] . Abar_pb[KP x NP]
+#pragma unroll
+      for (int t = 0; t < S::TPW; ++t) {
+        const int tile = wave * S::TPW + t;
+        if (tile < S::TC) {
+          const int tm = tile / (S::NP / 16), tn = tile % (S::NP / 16);
+          d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+          for (int ks = 0; ks < S::KP; ks += 4) {
+            const double a = Xs[(tm * 16 + (lane & 15)) * S::KP + ks + (lane >> 4)];
+            const double b = Apb[(ks + (lane >> 4)) * S::NP + tn * 16 + (lane & 15)];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+          }
+          for (int r = 0; r < 4; ++r)
+            Ts[(tm * 16 + (lane >> 4) + 4 * r) * S::NP + tn * 16 + (lane & 15)] = acc[r];
+        }
       }
       __syncthreads();
-      // Y[(k3,k4),(b3,b4)] += sum_{(k1,k2)} A[(k1,k2),(k3,k4),p] * Ts[(k1,k2),(b3,b4)]
+      const int p_lo = g.open_p ? 0 : pb, p_hi = g.open_p ? d : pb + 1;
+      for (int p = p_lo; p < p_hi; ++p) {
+        const double* Ap = Ac + (size_t)p * S::KP * S::NP;
+        if (g.open_p) {
 #pragma unroll
-      for (int i = 0; i < MAXPER; ++i) {
-        const int e = tid + i * NT;
-        if (e < D4) {
-          const int row = e / D2, col = e % D2;
-          double s = 0.0;
-          for (int kk = 0; kk < D2; ++kk) s += Ac[(kk * D2 + row) * d + p] * Ts[kk * D2 + col];
-          yacc[i] += s;
+          for (int t = 0; t < S::TPW; ++t) yacc[t] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+        // ---- step d: Y[(k3,k4), (blk,b3,b4)] += A_p^T[(k3,k4),(k1,k2)] . T[(blk,k1,k2),(b3,b4)]
+#pragma unroll
+        for (int t = 0; t < S::TPW; ++t) {
+          const int tile = wave * S::TPW + t;
+          if (tile < S::TC) {
+            const int tm = tile / (S::MP / 16), tn = tile % (S::MP / 16);
+            const int nn = tn * 16 + (lane & 15);
+            const int blk = nn / S::D2, b34 = nn % S::D2;
+            for (int ks = 0; ks < S::KP; ks += 4) {
+              const int kk = ks + (lane >> 4);
+              const double a = Ap[kk * S::NP + tm * 16 + (lane & 15)];
+              const double b =
+                  (kk < S::D2 && nn < S::ROWS) ? Ts[(blk * S::D2 + kk) * S::NP + b34] : 0.0;
+              yacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, yacc[t], 0, 0, 0);
+            }
+          }
+        }
+        if (g.open_p || (p == d - 1)) {
+          const int64_t poff = g.open_p ? (int64_t)p * g.y_p + (int64_t)pb * g.y_pb : 0;
+          // ---- write Y (after the last p when closed, per (p, pb) when open)
+#pragma unroll
+          for (int t = 0; t < S::TPW; ++t) {
+            const int tile = wave * S::TPW + t;
+            if (tile < S::TC) {
+              const int tm = tile / (S::MP / 16), tn = tile % (S::MP / 16);
+              const int nn = tn * 16 + (lane & 15);
+              const int blk = nn / S::D2, b34 = nn % S::D2;
+              if (nn < S::ROWS && b0 + blk < nblk) {
+                const int o1 = (b0 + blk) / g.n2, o2 = (b0 + blk) % g.n2;
+                const int64_t yb = (int64_t)o1 * g.y_o1 + (int64_t)o2 * g.y_o2 + poff;
+                const int b3 = b34 / D, b4 = b34 % D;
+                for (int r = 0; r < 4; ++r) {
+                  const int m = tm * 16 + (lane >> 4) + 4 * r;
+                  if (m < S::D2) {
+                    const int k3 = m / D, k4 = m % D;
+                    Y[yb + k3 * g.y_k3 + b3 * g.y_b3 + k4 * g.y_k4 + b4 * g.y_b4] = yacc[t][r];
+                  }
+                }
+              }
+            }
+          }
         }
       }
     }
-    const int64_t yb = (int64_t)o1 * g.y_o1 + (int64_t)o2 * g.y_o2;
-#pragma unroll
-    for (int i = 0; i < MAXPER; ++i) {
-      const int e = tid + i * NT;
-      if (e < D4) {
-        const int row = e / D2, col = e % D2;
-        const int k3 = row / D, k4 = row % D, b3 = col / D, b4 = col % D;
-        Y[yb + k3 * g.y_k3 + b3 * g.y_b3 + k4 * g.y_k4 + b4 * g.y_b4] = yacc[i];
-      }
-    }
   }
+}
+
+template <int D>
+void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hipStream_t s) {
+  using S = DLShape<D>;
+  static_assert(S::TC == (S::NP / 16) * (S::MP / 16), "tile counts of the two products differ");
+  const size_t lds = sizeof(double) * ((size_t)g.d * S::KP * S::NP + (size_t)S::MP * S::KP +
+                                       (size_t)S::MP * S::NP);
+  ACETN_REQUIRE(lds <= 160 * 1024, "double_layer: LDS budget exceeded");
+  static bool attr = false;
+  if (!attr) {
+    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)dl_mfma_kernel<D>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  const int64_t nblk = (int64_t)g.n1 * g.n2;
+  const int64_t ngroups = (nblk + S::NB - 1) / S::NB;
+  const int grid = (int)std::min<int64_t>(ngroups, 4096);
+  hipLaunchKernelGGL(dl_mfma_kernel<D>, dim3(grid), dim3(NT), lds, s, g, X, A, Y);
+  ACETN_HIP_CHECK(hipGetLastError());
 }
 }  // namespace
 
 void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
                       hipStream_t s) {
-  ACETN_REQUIRE(g.D >= 1 && g.D <= 8, "double_layer: D must be in [1, 8]");
-  ACETN_REQUIRE(g.d >= 1 && g.d <= 8, "double_layer: d must be in [1, 8]");
-  const int D4 = g.D * g.D * g.D * g.D;
-  const size_t lds = sizeof(double) * ((size_t)D4 * g.d + 2 * (size_t)D4);
-  ACETN_REQUIRE(lds <= 160 * 1024, "double_layer: LDS budget exceeded");
-  const int64_t nblk = (int64_t)g.n1 * g.n2;
-  if (nblk <= 0) return;
-  const int grid = (int)std::min<int64_t>(nblk, 2048);
-  hipLaunchKernelGGL(double_layer_kernel, dim3(grid), dim3(NT), lds, s, g, X, A, Y);
-  ACETN_HIP_CHECK(hipGetLastError());
+  ACETN_REQUIRE(g.d >= 1 && g.d <= 4, "double_layer: d must be in [1, 4]");
+  if ((int64_t)g.n1 * g.n2 <= 0) return;
+  switch (g.D) {
+    case 1: launch_dl<1>(g, X, A, Y, s); break;
+    case 2: launch_dl<2>(g, X, A, Y, s); break;
+    case 3: launch_dl<3>(g, X, A, Y, s); break;
+    case 4: launch_dl<4>(g, X, A, Y, s); break;
+    case 5: launch_dl<5>(g, X, A, Y, s); break;
+    case 6: launch_dl<6>(g, X, A, Y, s); break;
+    case 7: launch_dl<7>(g, X, A, Y, s); break;
+    case 8: launch_dl<8>(g, X, A, Y, s); break;
+    default: ACETN_REQUIRE(false, "double_layer: D must be in [1, 8]");
+  }
 }
 
 }  // namespace acetn

@@ -5,8 +5,8 @@
 // permutations the reference performs with torch.einsum transposes are folded
 // into the global->LDS tile loads instead of being materialised in HBM.
 //
-// Tile: 64 x 64 per workgroup, BK = 16, 4 waves in a 2 x 2 arrangement, each
-// wave owning a 32 x 32 sub-tile = 2 x 2 MFMA blocks of 16 x 16.  Operand
+// Tiles: 4 waves in a 2 x 2 arrangement, each owning WM x WN MFMA blocks of
+// 16 x 16 (workgroup tile 32*WM x 32*WN, chosen per shape), BK = 16.  Operand
 // fragments follow the MFMA register layout: A(16x4) lane l holds
 // A[l%16][l/16], B(4x16) lane l holds B[l/16][l%16], and the 16x16 f64
 // result lane l holds D[l/16 + 4r][l%16], r = 0..3 (checked on gfx950).
@@ -14,21 +14,70 @@
 // buffer that a second kernel reduces (no atomics, bitwise reproducible).
 #include "internal.h"
 
+#include <mutex>
+#include <vector>
+
 namespace acetn {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// ---- optional live timing of every gemm_f64_kernel launch (HIP events on
+// the launch stream), used by the benchmark for the roofline figure.
 namespace {
-constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
-constexpr int LDS_A = BM + 2;
-constexpr int LDS_B = BN + 2;
+struct KTimer {
+  std::mutex mu;
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  double flops = 0.0;
+  int64_t launches = 0;
+};
+KTimer& ktimer() {
+  static KTimer t;
+  return t;
+}
+}  // namespace
 
-__device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s_in == 1; }
+void kernel_timing_enable(bool on) {
+  KTimer& t = ktimer();
+  std::lock_guard<std::mutex> g(t.mu);
+  for (auto& e : t.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  t.ev.clear();
+  t.flops = 0.0;
+  t.launches = 0;
+  t.on = on;
+}
 
+void kernel_timing_read(double* ms, double* flops, int64_t* launches) {
+  KTimer& t = ktimer();
+  std::lock_guard<std::mutex> g(t.mu);
+  double tot = 0.0;
+  for (auto& e : t.ev) {
+    ACETN_HIP_CHECK(hipEventSynchronize(e.second));
+    float x = 0.f;
+    ACETN_HIP_CHECK(hipEventElapsedTime(&x, e.first, e.second));
+    tot += x;
+  }
+  *ms = tot;
+  *flops = t.flops;
+  *launches = t.launches;
+}
+
+namespace {
+constexpr int BK = 16, NT = 256;
+
+__device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; }
+
+// Workgroup tile BM x BN with BM = 32*WM, BN = 32*WN: 4 waves in a 2 x 2
+// grid, each wave owning WM x WN MFMA blocks of 16 x 16.  Double-buffered LDS
+// (one barrier per k-tile) with the next tile prefetched into registers.
+template <int WM, int WN>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, int ksplit,
                                                       double* __restrict__ work) {
-  __shared__ double As[BK][LDS_A];
-  __shared__ double Bs[BK][LDS_B];
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int LA = BM + 2, LB = BN + 2;
+  constexpr int AE = BM * BK / NT, BE = BN * BK / NT;  // elements per thread
+  __shared__ double As[2][BK][LA];
+  __shared__ double Bs[2][BK][LB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -42,74 +91,97 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
   const int kbeg = kpart * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
 
-  // ---- load pattern for A (64 x 16 tile, 4 elements per thread)
+  // ---- load patterns (coalesced along whichever index is unit-stride)
   const bool a_mfast = contiguous(g.am) || !contiguous(g.ak);
-  int a_ml[4], a_kl[4];
-  int64_t a_moff[4];
-  bool a_mok[4];
-  for (int i = 0; i < 4; ++i) {
-    if (a_mfast) { a_ml[i] = tid % BM; a_kl[i] = tid / BM + 4 * i; }
-    else         { a_kl[i] = tid % BK; a_ml[i] = tid / BK + 16 * i; }
+  int a_ml[AE], a_kl[AE];
+  int64_t a_moff[AE];
+  bool a_mok[AE];
+#pragma unroll
+  for (int i = 0; i < AE; ++i) {
+    const int e = tid + i * NT;
+    if (a_mfast) { a_ml[i] = e % BM; a_kl[i] = e / BM; }
+    else         { a_kl[i] = e % BK; a_ml[i] = e / BK; }
     const int m = m0 + a_ml[i];
     a_mok[i] = m < g.M;
     a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
   }
-  // ---- load pattern for B (16 x 64 tile)
   const bool b_nfast = contiguous(g.bn) || !contiguous(g.bk);
-  int b_nl[4], b_kl[4];
-  int64_t b_noff[4];
-  bool b_nok[4];
-  for (int i = 0; i < 4; ++i) {
-    if (b_nfast) { b_nl[i] = tid % BN; b_kl[i] = tid / BN + 4 * i; }
-    else         { b_kl[i] = tid % BK; b_nl[i] = tid / BK + 16 * i; }
+  int b_nl[BE], b_kl[BE];
+  int64_t b_noff[BE];
+  bool b_nok[BE];
+#pragma unroll
+  for (int i = 0; i < BE; ++i) {
+    const int e = tid + i * NT;
+    if (b_nfast) { b_nl[i] = e % BN; b_kl[i] = e / BN; }
+    else         { b_kl[i] = e % BK; b_nl[i] = e / BK; }
     const int n = n0 + b_nl[i];
     b_nok[i] = n < g.N;
     b_noff[i] = b_nok[i] ? imap(g.bn, n) : 0;
   }
 
-  d4 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  d4 acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rb[4];
+  double ra[AE], rb[BE];
   auto load_tile = [&](int kt) {
-    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
       const int k = kt + a_kl[i];
       ra[i] = (a_mok[i] && k < kend) ? A[a_moff[i] + imap(g.ak, k)] : 0.0;
     }
-    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
       const int k = kt + b_kl[i];
       rb[i] = (b_nok[i] && k < kend) ? B[imap(g.bk, k) + b_noff[i]] : 0.0;
     }
   };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) As[buf][a_kl[i]][a_ml[i]] = ra[i];
+#pragma unroll
+    for (int i = 0; i < BE; ++i) Bs[buf][b_kl[i]][b_nl[i]] = rb[i];
+  };
 
-  if (kbeg < kend) load_tile(kbeg);
+  int buf = 0;
+  if (kbeg < kend) {
+    load_tile(kbeg);
+    store_tile(0);
+  }
+  __syncthreads();
   for (int kt = kbeg; kt < kend; kt += BK) {
-    __syncthreads();
-    for (int i = 0; i < 4; ++i) As[a_kl[i]][a_ml[i]] = ra[i];
-    for (int i = 0; i < 4; ++i) Bs[b_kl[i]][b_nl[i]] = rb[i];
-    __syncthreads();
-    if (kt + BK < kend) load_tile(kt + BK);  // prefetch next tile into registers
+    const bool more = kt + BK < kend;
+    if (more) load_tile(kt + BK);  // global loads in flight during the MFMAs
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
-      double a0 = As[kr][wm * 32 + (lane & 15)];
-      double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
-      double b0 = Bs[kr][wn * 32 + (lane & 15)];
-      double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      double a[WM], b[WN];
+#pragma unroll
+      for (int i = 0; i < WM; ++i) a[i] = As[buf][kr][wm * 16 * WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < WN; ++j) b[j] = Bs[buf][kr][wn * 16 * WN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (more) store_tile(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   // ---- epilogue
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
-        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        const int m = m0 + wm * 16 * WM + i * 16 + (lane >> 4) + 4 * r;
+        const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
         if (m >= g.M || n >= g.N) continue;
         const double v = acc[i][j][r];
         if (ksplit == 1) {
@@ -137,13 +209,34 @@ __global__ void splitk_reduce_kernel(GemmDesc g, GemmBatch bt, int ksplit,
   }
 }
 
+struct Tile {
+  int wm, wn;
+};
+// per-shape tile: skinny range-finder products (N <= 160) use 64 x 160 / 32 x 160,
+// large products 128 x 128, small ones 64 x 64
+Tile choose_tile(const GemmDesc& g) {
+  if (g.N > 96 && g.N <= 160) return g.M >= 2048 ? Tile{2, 5} : Tile{1, 5};
+  if (g.M >= 1024 && g.N >= 1024) return Tile{4, 4};
+  return Tile{2, 2};
+}
+
 int choose_ksplit(const GemmDesc& g, int count) {
+  const Tile t = choose_tile(g);
+  const int BM = 32 * t.wm, BN = 32 * t.wn;
   const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * count;
   if (tiles >= 512 || g.K < 4 * BK * 8) return 1;
   int ks = (int)((1024 + tiles - 1) / tiles);
   ks = std::min(ks, g.K / (BK * 8));
   ks = std::min(ks, 64);
   return std::max(ks, 1);
+}
+
+template <int WM, int WN>
+void launch_gemm(const GemmDesc& g, const GemmBatch& b, int ks, double* work, hipStream_t s) {
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, b.count * ks);
+  ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_f64: grid too large");
+  hipLaunchKernelGGL((gemm_f64_kernel<WM, WN>), grid, dim3(NT), 0, s, g, b, ks, work);
 }
 }  // namespace
 
@@ -162,10 +255,26 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
   }
   int ks = choose_ksplit(g, b.count);
   if (ks > 1 && (work == nullptr || work_elems < (size_t)ks * b.count * g.M * g.N)) ks = 1;
-  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, b.count * ks);
-  ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_f64: grid too large");
-  hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(NT), 0, s, g, b, ks, work);
+  const Tile t = choose_tile(g);
+  KTimer& kt = ktimer();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (kt.on) {
+    ACETN_HIP_CHECK(hipEventCreate(&e0));
+    ACETN_HIP_CHECK(hipEventCreate(&e1));
+    ACETN_HIP_CHECK(hipEventRecord(e0, s));
+  }
+  if (t.wm == 2 && t.wn == 5) launch_gemm<2, 5>(g, b, ks, work, s);
+  else if (t.wm == 1 && t.wn == 5) launch_gemm<1, 5>(g, b, ks, work, s);
+  else if (t.wm == 4 && t.wn == 4) launch_gemm<4, 4>(g, b, ks, work, s);
+  else launch_gemm<2, 2>(g, b, ks, work, s);
   ACETN_HIP_CHECK(hipGetLastError());
+  if (kt.on) {
+    ACETN_HIP_CHECK(hipEventRecord(e1, s));
+    std::lock_guard<std::mutex> lg(kt.mu);
+    kt.ev.emplace_back(e0, e1);
+    kt.flops += 2.0 * g.M * g.N * (double)g.K * b.count;
+    kt.launches += 1;
+  }
   if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);

@@ -31,24 +31,44 @@ struct Error : std::runtime_error {
   } while (0)
 
 // ---------------------------------------------------------------------------
-// Two-level index map: offset(i) = (i % inner) * s_in + (i / inner) * s_out.
-// inner <= 0 means a plain stride: offset(i) = i * s_in.  This lets a GEMM
-// operand index a grouped einsum mode (e.g. (e, l, L) of an edge tensor laid
-// out as [e][a][l][L]) without materialising the permutation.
+// Multi-level index map: a flat GEMM index i is split into up to four mixed-
+// radix digits (innermost first, radices n0, n1, n2; the last digit takes the
+// rest), each with its own stride:
+//   offset(i) = (i % n0)*s0 + ((i/n0) % n1)*s1 + ((i/(n0 n1)) % n2)*s2
+//               + (i/(n0 n1 n2))*s3
+// n0 <= 0 means a plain stride: offset(i) = i*s0.  This lets a GEMM operand
+// index a grouped einsum mode (e.g. (e, l, L) of an edge tensor laid out as
+// [e][a][l][L]) without materialising the permutation.
 // ---------------------------------------------------------------------------
 struct IdxMap {
-  int32_t inner;
-  int64_t s_in;
-  int64_t s_out;
+  int32_t n0, n1, n2;
+  int64_t s0, s1, s2, s3;
 };
-inline IdxMap plain(int64_t s) { return IdxMap{0, s, 0}; }
+inline IdxMap plain(int64_t s) { return IdxMap{0, 0, 0, s, 0, 0, 0}; }
+// two digits: (i % inner)*s_in + (i / inner)*s_out
 inline IdxMap grouped(int32_t inner, int64_t s_in, int64_t s_out) {
-  return IdxMap{inner, s_in, s_out};
+  return IdxMap{inner, 0, 0, s_in, s_out, 0, 0};
+}
+// three digits
+inline IdxMap grouped3(int32_t n0, int64_t s0, int32_t n1, int64_t s1, int64_t s2) {
+  return IdxMap{n0, n1, 0, s0, s1, s2, 0};
+}
+// four digits
+inline IdxMap grouped4(int32_t n0, int64_t s0, int32_t n1, int64_t s1, int32_t n2, int64_t s2,
+                       int64_t s3) {
+  return IdxMap{n0, n1, n2, s0, s1, s2, s3};
 }
 
 __device__ __forceinline__ int64_t imap(const IdxMap& m, int32_t i) {
-  if (m.inner <= 0) return (int64_t)i * m.s_in;
-  return (int64_t)(i % m.inner) * m.s_in + (int64_t)(i / m.inner) * m.s_out;
+  if (m.n0 <= 0) return (int64_t)i * m.s0;
+  int64_t off = (int64_t)(i % m.n0) * m.s0;
+  int32_t r = i / m.n0;
+  if (m.n1 <= 0) return off + (int64_t)r * m.s1;
+  off += (int64_t)(r % m.n1) * m.s1;
+  r /= m.n1;
+  if (m.n2 <= 0) return off + (int64_t)r * m.s2;
+  off += (int64_t)(r % m.n2) * m.s2;
+  return off + (int64_t)(r / m.n2) * m.s3;
 }
 
 constexpr int kMaxBatch = 16;
@@ -75,6 +95,9 @@ struct GemmBatch {
 void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_elems,
               hipStream_t s);
 size_t gemm_f64_work_elems(const GemmDesc& g, int count);
+// Live timing of gemm_f64_kernel launches (events on the launch stream).
+void kernel_timing_enable(bool on);
+void kernel_timing_read(double* ms, double* flops, int64_t* launches);
 
 // Convenience: dense row-major C = alpha*op(A) op(B) + beta*C.
 // A is (M x K) if !ta else stored (K x M); lda/ldb/ldc are row strides.
@@ -87,12 +110,17 @@ GemmDesc dense_desc(int M, int N, int K, bool ta, int64_t lda, bool tb, int64_t 
 //                           * A[k1,k2,k3,k4,p] * conj(A)[b1,b2,b3,b4,p]
 // A's legs are addressed through strides, which folds bond_permute in.
 // ---------------------------------------------------------------------------
+// With open_p != 0 the physical indices stay open: Y gets the two extra
+// indices p (ket, stride y_p) and pb (bra, stride y_pb) and nothing is summed
+// over them (the bond reduced density matrix, acetn/measurement/rdm.py:102-120).
 struct DLDesc {
   int32_t D, d, n1, n2;
   int64_t x_o1, x_o2, x_k1, x_b1, x_k2, x_b2;
   int64_t y_o1, y_o2, y_k3, y_b3, y_k4, y_b4;
   int64_t a_leg[4];
   int64_t a_p;
+  int32_t open_p;
+  int64_t y_p, y_pb;
 };
 void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
                       hipStream_t s);
@@ -108,8 +136,11 @@ void gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed, uint64_t 
 // (row-major, ld=q), write the upper-triangular inverse factor Rinv (q x q,
 // row-major) with G + shift*I = R^T R.  Columns whose pivot falls below
 // rank_tol * max(diag G) are marked dead: their rows/cols of Rinv are zero.
+// ``only_if_dead`` (device, q ints): when given and all zero, the kernel
+// skips the factorisation and writes Rinv = I (completion passes).
 void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double rank_tol,
-               hipStream_t s, double* gscratch, int* dead_out = nullptr);
+               hipStream_t s, double* gscratch, int* dead_out = nullptr,
+               const int* only_if_dead = nullptr);
 // Replace the columns flagged in ``dead`` (device, q ints) of Y (m x q,
 // row-major) by N(0,1) Philox vectors.
 void refill_dead_columns(double* Y, int64_t m, int q, const int* dead, uint64_t seed,
@@ -119,7 +150,10 @@ void refill_dead_columns(double* Y, int64_t m, int q, const int* dead, uint64_t 
 // row-major) is overwritten; eigenvalues (descending) go to ``w`` and
 // eigenvectors (columns, row-major q x q) to ``V``.  ``rot`` is scratch for
 // the recorded rotations (see jacobi_rot_elems).
-void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s);
+// ``tol``: a pair is rotated while |g_pq| > tol * sqrt(g_pp g_qq); 2.2e-16
+// gives a converged decomposition, a looser value a preconditioning one.
+void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s,
+                 double tol = 2.2e-16);
 size_t jacobi_rot_elems(int q);
 
 // Column scaling / projector finishing kernels.

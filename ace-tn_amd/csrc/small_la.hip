@@ -16,7 +16,8 @@
 namespace acetn {
 namespace {
 
-constexpr int LA_NT = 1024;
+constexpr int LA_NT = 256;     // Cholesky: latency-bound steps, cheap barriers
+constexpr int JAC_NT = 1024;   // Jacobi: LDS-latency-bound, needs many waves
 constexpr size_t kLdsBudget = 150 * 1024;
 
 __device__ __forceinline__ int pk(int i, int j) {  // packed upper, i <= j required
@@ -65,14 +66,27 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
                                                           double* __restrict__ Rinv, int q,
                                                           double shift_rel, double rank_tol,
                                                           double* gscratch, int use_lds,
-                                                          int* dead_out) {
+                                                          int* dead_out, const int* only_if_dead) {
   extern __shared__ double lds[];
   double* S = use_lds ? lds : gscratch;
+  if (only_if_dead) {
+    // completion pass: nothing to do unless an earlier pass refilled a column
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < q; i += LA_NT)
+      if (only_if_dead[i]) any = 1;
+    __syncthreads();
+    if (!any) {
+      for (int e = threadIdx.x; e < q * q; e += LA_NT) Rinv[e] = (e / q == e % q) ? 1.0 : 0.0;
+      return;
+    }
+  }
   __shared__ double s_red[LA_NT / 64];
   __shared__ double s_trace, s_dmax;
   __shared__ int dead[512];
+  __shared__ double tmp[512];
   const int tid = threadIdx.x;
-  const int np = q * (q + 1) / 2;
 
   // trace and max diagonal
   double tr = 0.0, dm = 0.0;
@@ -89,19 +103,14 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
   const double shift = shift_rel * s_trace;
   const double thresh = rank_tol * (s_dmax + shift);
 
-  for (int e = tid; e < np; e += LA_NT) {
-    // invert pk: find j with j(j+1)/2 <= e
-    int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-    while ((j + 1) * (j + 2) / 2 <= e) ++j;
-    while (j * (j + 1) / 2 > e) --j;
-    const int i = e - j * (j + 1) / 2;
-    double v = G[(int64_t)i * q + j];
-    if (i == j) v += shift;
-    S[e] = v;
+  // packed upper triangle of G + shift*I
+  for (int e = tid; e < q * q; e += LA_NT) {
+    const int i = e / q, j = e % q;
+    if (i <= j) S[pk(i, j)] = G[(int64_t)i * q + j] + (i == j ? shift : 0.0);
   }
-  for (int i = tid; i < q; i += LA_NT) dead[i] = 0;
   __syncthreads();
 
+  // right-looking Cholesky, G = R^T R (R upper, stored in place)
   for (int j = 0; j < q; ++j) {
     const double piv = S[pk(j, j)];
     const bool is_dead = !(piv > thresh) || !isfinite(piv);
@@ -111,33 +120,51 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
     for (int k = j + 1 + tid; k < q; k += LA_NT) S[pk(j, k)] = is_dead ? 0.0 : S[pk(j, k)] / rjj;
     __syncthreads();
     if (!is_dead) {
-      const int m = q - j - 1;  // trailing size
-      const int nt = m * (m + 1) / 2;
-      for (int e = tid; e < nt; e += LA_NT) {
-        int kk = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-        while ((kk + 1) * (kk + 2) / 2 <= e) ++kk;
-        while (kk * (kk + 1) / 2 > e) --kk;
-        const int ii = e - kk * (kk + 1) / 2;
-        const int i = j + 1 + ii, k = j + 1 + kk;
-        S[pk(i, k)] -= S[pk(j, i)] * S[pk(j, k)];
+      // rows ii by thread groups of 32, columns kk >= ii strided within the group
+      const int m = q - j - 1;
+      for (int ii = tid >> 5; ii < m; ii += LA_NT / 32) {
+        const int i = j + 1 + ii;
+        const double rji = S[pk(j, i)];
+        for (int kk = ii + (tid & 31); kk < m; kk += 32) {
+          const int k = j + 1 + kk;
+          S[pk(i, k)] -= rji * S[pk(j, k)];
+        }
       }
     }
     __syncthreads();
   }
-
   if (dead_out)
     for (int i = tid; i < q; i += LA_NT) dead_out[i] = dead[i];
-  // Rinv columns by back substitution; one thread per column.
-  for (int c = tid; c < q; c += LA_NT) {
-    for (int i = 0; i < q; ++i) Rinv[(int64_t)i * q + c] = 0.0;
-    if (dead[c]) continue;
-    Rinv[(int64_t)c * q + c] = 1.0 / S[pk(c, c)];
-    for (int i = c - 1; i >= 0; --i) {
-      if (dead[i]) continue;
-      double acc = 0.0;
-      for (int k = i + 1; k <= c; ++k) acc += S[pk(i, k)] * Rinv[(int64_t)k * q + c];
-      Rinv[(int64_t)i * q + c] = -acc / S[pk(i, i)];
+
+  // in-place inverse of the packed upper factor, column by column:
+  // Rinv[0:j, j] = -Rinv[0:j, 0:j] R[0:j, j] / R[j, j].  Dead columns/rows are 0.
+  for (int j = 0; j < q; ++j) {
+    const bool dj = dead[j] != 0;
+    for (int i = tid; i < j; i += LA_NT) tmp[i] = S[pk(i, j)];
+    __syncthreads();
+    if (dj) {
+      for (int i = tid; i <= j; i += LA_NT) S[pk(i, j)] = 0.0;
+    } else {
+      const double inv = 1.0 / S[pk(j, j)];
+      // 8 lanes per row i, each summing a strided part of k in [i, j)
+      const int g = tid & 7;
+      for (int i0 = 0; i0 < j; i0 += LA_NT / 8) {
+        const int i = i0 + (tid >> 3);
+        double y = 0.0;
+        if (i < j && !dead[i])
+          for (int k = i + g; k < j; k += 8) y += S[pk(i, k)] * tmp[k];
+        y += __shfl_xor(y, 1);
+        y += __shfl_xor(y, 2);
+        y += __shfl_xor(y, 4);
+        if (i < j && g == 0) S[pk(i, j)] = -y * inv;
+      }
+      if (tid == 0) S[pk(j, j)] = inv;
     }
+    __syncthreads();
+  }
+  for (int e = tid; e < q * q; e += LA_NT) {
+    const int i = e / q, j = e % q;
+    Rinv[e] = (i <= j) ? S[pk(i, j)] : 0.0;
   }
 }
 
@@ -178,20 +205,22 @@ __device__ __forceinline__ void round_pair(int n, int r, int a, int& p, int& qq)
 
 constexpr int kMaxSweeps = 40;
 
-__global__ __launch_bounds__(LA_NT) void jacobi_kernel(const double* __restrict__ Gin, int q,
+__global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict__ Gin, int q,
                                                        double* __restrict__ w,
                                                        double* __restrict__ rot, int* info,
-                                                       double* gscratch, int use_lds) {
+                                                       double* gscratch, int use_lds,
+                                                       double tol) {
   extern __shared__ double lds[];
   double* S = use_lds ? lds : gscratch;
   __shared__ double cs_c[256], cs_s[256];
+  __shared__ short pp[256], qq_[256];
   __shared__ int any_rot;
   const int tid = threadIdx.x;
   const int n = q + (q & 1);
   const int half = n / 2;
   const int np = n * (n + 1) / 2;
 
-  for (int e = tid; e < np; e += LA_NT) {
+  for (int e = tid; e < np; e += JAC_NT) {
     int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
     while ((j + 1) * (j + 2) / 2 <= e) ++j;
     while (j * (j + 1) / 2 > e) --j;
@@ -201,20 +230,32 @@ __global__ __launch_bounds__(LA_NT) void jacobi_kernel(const double* __restrict_
   __syncthreads();
 
   const int nblocks = half * (half + 1) / 2;
+  // this thread's share of the 2x2 blocks (a <= b) in triangular order
+  // blk = b(b+1)/2 + a: start at blk = tid, advance by JAC_NT per block
+  int a_start, b_start;
+  {
+    int b = (int)((sqrt(8.0 * tid + 1.0) - 1.0) * 0.5);
+    while ((b + 1) * (b + 2) / 2 <= tid) ++b;
+    while (b * (b + 1) / 2 > tid) --b;
+    b_start = b;
+    a_start = tid - b * (b + 1) / 2;
+  }
   int sweep = 0;
   for (; sweep < kMaxSweeps; ++sweep) {
     __syncthreads();  // every thread has read the previous sweep's flag
     if (tid == 0) any_rot = 0;
     __syncthreads();
     for (int r = 0; r < n - 1; ++r) {
-      // rotation parameters
-      for (int a = tid; a < half; a += LA_NT) {
+      // rotation parameters (and this round's pair table)
+      for (int a = tid; a < half; a += JAC_NT) {
         int p, qq;
         round_pair(n, r, a, p, qq);
+        pp[a] = (short)p;
+        qq_[a] = (short)qq;
         double c = 1.0, s = 0.0;
         if (qq < q) {
           const double app = S[pk(p, p)], aqq = S[pk(qq, qq)], apq = S[pk(p, qq)];
-          if (apq != 0.0 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq))) {
+          if (apq != 0.0 && fabs(apq) > tol * sqrt(fabs(app * aqq))) {
             const double tau = (aqq - app) / (2.0 * apq);
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -228,14 +269,13 @@ __global__ __launch_bounds__(LA_NT) void jacobi_kernel(const double* __restrict_
       }
       __syncthreads();
       // apply J^T G J on every 2x2 block (a <= b)
-      for (int blk = tid; blk < nblocks; blk += LA_NT) {
-        int b = (int)((sqrt(8.0 * blk + 1.0) - 1.0) * 0.5);
-        while ((b + 1) * (b + 2) / 2 <= blk) ++b;
-        while (b * (b + 1) / 2 > blk) --b;
-        const int a = blk - b * (b + 1) / 2;
-        int pa, qa, pb, qb;
-        round_pair(n, r, a, pa, qa);
-        round_pair(n, r, b, pb, qb);
+      int a = a_start, b = b_start;
+      for (int blk = tid; blk < nblocks; blk += JAC_NT) {
+        if (blk != tid) {
+          a += JAC_NT;
+          while (a > b) { a -= b + 1; ++b; }
+        }
+        const int pa = pp[a], qa = qq_[a], pb = pp[b], qb = qq_[b];
         const double ca = cs_c[a], sa = cs_s[a], cb = cs_c[b], sb = cs_s[b];
         if (a == b) {
           if (sa == 0.0) continue;
@@ -262,21 +302,25 @@ __global__ __launch_bounds__(LA_NT) void jacobi_kernel(const double* __restrict_
     if (!rotated) { ++sweep; break; }
   }
   if (tid == 0) info[0] = sweep;
-  for (int i = tid; i < q; i += LA_NT) w[i] = S[pk(i, i)];
+  for (int i = tid; i < q; i += JAC_NT) w[i] = S[pk(i, i)];
 }
 
 // Replay the recorded rotations on the rows of V (V starts as identity).
 // Each workgroup owns ``rows`` rows in LDS.  Afterwards the eigenpairs are
 // sorted by descending eigenvalue (ties: lower index first).
+constexpr int kReplayRounds = 16;
+
 __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __restrict__ rot,
                                                              const int* __restrict__ info, int q,
                                                              int rows, double* __restrict__ w,
                                                              double* __restrict__ V,
                                                              const double* __restrict__ wraw) {
-  extern __shared__ double vr[];  // rows x n
+  extern __shared__ double vr[];  // rows x n, then kReplayRounds x half x 2 rotations
   __shared__ int perm[512];
+  __shared__ short pp[kReplayRounds][256], qq_[kReplayRounds][256];
   const int n = q + (q & 1);
   const int half = n / 2;
+  double* rs = vr + rows * n;
   const int r0 = blockIdx.x * rows;
   const int nr = min(rows, q - r0);
   const int tid = threadIdx.x;
@@ -284,23 +328,33 @@ __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __res
     const int rr = e / n, cc = e % n;
     vr[e] = (rr < nr && r0 + rr == cc) ? 1.0 : 0.0;
   }
-  const int sweeps = info[0];
-  __syncthreads();
-  for (int st = 0; st < sweeps * (n - 1); ++st) {
-    const int r = st % (n - 1);
-    for (int e = tid; e < half * nr; e += blockDim.x) {
-      const int a = e % half, rr = e / half;
-      const int64_t ro = ((int64_t)st * half + a) * 2;
-      const double c = rot[ro], s = rot[ro + 1];
-      if (s == 0.0) continue;
-      int p, qq;
-      round_pair(n, r, a, p, qq);
-      double* row = vr + rr * n;
-      const double vp = row[p], vq = row[qq];
-      row[p] = c * vp - s * vq;
-      row[qq] = s * vp + c * vq;
+  const int total = info[0] * (n - 1);
+  for (int st0 = 0; st0 < total; st0 += kReplayRounds) {
+    const int nrounds = min(kReplayRounds, total - st0);
+    __syncthreads();
+    for (int e = tid; e < nrounds * half * 2; e += blockDim.x)
+      rs[e] = rot[(int64_t)st0 * half * 2 + e];
+    for (int e = tid; e < nrounds * half; e += blockDim.x) {
+      const int rr = e / half, a = e % half;
+      int p, qv;
+      round_pair(n, (st0 + rr) % (n - 1), a, p, qv);
+      pp[rr][a] = (short)p;
+      qq_[rr][a] = (short)qv;
     }
     __syncthreads();
+    for (int rr = 0; rr < nrounds; ++rr) {
+      for (int e = tid; e < half * nr; e += blockDim.x) {
+        const int a = e % half, row = e / half;
+        const double c = rs[(rr * half + a) * 2], sn = rs[(rr * half + a) * 2 + 1];
+        if (sn == 0.0) continue;
+        double* v = vr + row * n;
+        const int p = pp[rr][a], qv = qq_[rr][a];
+        const double vp = v[p], vq = v[qv];
+        v[p] = c * vp - sn * vq;
+        v[qv] = sn * vp + c * vq;
+      }
+      __syncthreads();
+    }
   }
   // descending order permutation from the raw eigenvalues
   for (int i = tid; i < q; i += blockDim.x) {
@@ -421,13 +475,13 @@ void gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed, uint64_t 
 }
 
 void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double rank_tol,
-               hipStream_t s, double* gscratch, int* dead_out) {
+               hipStream_t s, double* gscratch, int* dead_out, const int* only_if_dead) {
   ACETN_REQUIRE(q >= 1 && q <= 512, "chol_rinv: q out of range");
   const size_t pb = packed_bytes(q);
   const int use_lds = pb <= kLdsBudget;
   ACETN_REQUIRE(use_lds || gscratch != nullptr, "chol_rinv: scratch required");
   hipLaunchKernelGGL(chol_rinv_kernel, dim3(1), dim3(LA_NT), use_lds ? pb : 0, s, G, Rinv, q,
-                     shift_rel, rank_tol, gscratch, use_lds, dead_out);
+                     shift_rel, rank_tol, gscratch, use_lds, dead_out, only_if_dead);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 
@@ -444,7 +498,7 @@ size_t jacobi_rot_elems(int q) {
   return (size_t)kMaxSweeps * (n - 1) * (n / 2) * 2 + (size_t)q + 8 + (size_t)n * (n + 1) / 2;
 }
 
-void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s) {
+void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s, double tol) {
   ACETN_REQUIRE(q >= 1 && q <= 510, "jacobi_eigh: q out of range");
   const int n = q + (q & 1);
   double* wraw = rot + (size_t)kMaxSweeps * (n - 1) * (n / 2) * 2;
@@ -452,13 +506,14 @@ void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_
   double* gscr = wraw + q + 8;
   const size_t pb = packed_bytes(n);
   const int use_lds = pb <= kLdsBudget;
-  hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(LA_NT), use_lds ? pb : 0, s, G, q, wraw, rot,
-                     info, gscr, use_lds);
+  hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(JAC_NT), use_lds ? pb : 0, s, G, q, wraw, rot,
+                     info, gscr, use_lds, tol);
   ACETN_HIP_CHECK(hipGetLastError());
   const int rows = 8;
   const int nblk = (q + rows - 1) / rows;
-  hipLaunchKernelGGL(jacobi_vectors_kernel, dim3(nblk), dim3(256), sizeof(double) * rows * n, s,
-                     rot, info, q, rows, w, V, wraw);
+  const size_t lds = sizeof(double) * ((size_t)rows * n + (size_t)kReplayRounds * (n / 2) * 2);
+  hipLaunchKernelGGL(jacobi_vectors_kernel, dim3(nblk), dim3(256), lds, s, rot, info, q, rows, w,
+                     V, wraw);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

@@ -49,6 +49,15 @@ const char* acetn_hip_last_error(void);
 /* Library / device self-description: writes up to ``len`` bytes. */
 int acetn_hip_device_info(char* host_buf, size_t len);
 
+/* Measurement hook (no reference counterpart; it serves the timing
+ * convention of acetn/utils/benchmarking.py:45-104): when enabled, every
+ * launch of the FP64 matrix-core GEMM kernel is bracketed by HIP events on its
+ * stream; _read() synchronises and returns the summed kernel time (ms), the
+ * summed algorithmic flops (2*M*N*K per launch) and the launch count.
+ * Enabling (or disabling) resets the counters. */
+int acetn_hip_kernel_timing(int enable);
+int acetn_hip_kernel_timing_read(double* host_ms, double* host_flops, int64_t* host_launches);
+
 /* Enlarged corner ("quarter tensor") Q_k of one site, as an
  * (C_dim[k][1]*D^2) x (E_dim[(3+k)%4][0]*D^2) matrix.
  * Replaces ProjectorCalculator.make_quarter_tensor,
@@ -75,6 +84,24 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
                               int64_t* host_cD_new, double* spectrum, void* workspace,
                               size_t ws_bytes, void* stream);
 
+/* The same computation split at the host readback of cD_new, so that the
+ * independent projector pairs of a stage (the left/right or up/down move
+ * pair of acetn/renormalization/ctmrg.py:26-31) can run concurrently on
+ * separate streams with one host synchronisation per stage:
+ *   _begin enqueues everything through the truncation count and an async
+ *   copy of cD_new into ``pinned_cD_new`` (pinned host memory);
+ *   _end (after the host has synchronised the stream and read cD_new)
+ *   enqueues the projector products into P1/P2.  Both calls must receive
+ *   the same sites, dimensions and workspace (its contents carry over). */
+int acetn_hip_half_projectors_begin(const acetn_site* s1, const acetn_site* s4, int k,
+                                    int64_t chi, int64_t oversampling, int niter,
+                                    double svd_cutoff, uint64_t seed, uint64_t sketch_stream,
+                                    int32_t* pinned_cD_new, double* spectrum, void* workspace,
+                                    size_t ws_bytes, void* stream);
+int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
+                                  int64_t oversampling, int niter, int64_t cD_new, double* P1,
+                                  double* P2, void* workspace, size_t ws_bytes, void* stream);
+
 /* Boundary renormalisation of one (s1 -> s2) target.
  * Replaces DirectionalMover.renormalize_boundary / renormalize_cj1 /
  * renormalize_cj2 / renormalize_ej, acetn/renormalization/directional_mover.py:273-366.
@@ -89,6 +116,20 @@ int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1
                                    const double* P2_i, int64_t cD_i, const double* P1_j,
                                    const double* P2_j, int64_t cD_j, double* C_a, double* C_b,
                                    double* E_o, void* workspace, size_t ws_bytes, void* stream);
+
+/* One-site reduced density matrix rho[P][p] (d x d) of a site from its
+ * C[0..3], E[0..3] and A.  Replaces RDM.build_site_rdm,
+ * acetn/measurement/rdm.py:35-67. */
+size_t acetn_hip_site_rdm_workspace(const acetn_site* s);
+int acetn_hip_site_rdm(const acetn_site* s, double* rho, void* workspace, size_t ws_bytes,
+                       void* stream);
+
+/* Bond reduced density matrix rho[P][Q][p][q] (d,d,d,d) of the bond (s1, s2, k)
+ * in s1's bond frame.  Replaces RDM.build_bond_rdm / build_bond_rdm_core(_blocked),
+ * acetn/measurement/rdm.py:69-154. */
+size_t acetn_hip_bond_rdm_workspace(const acetn_site* s1, const acetn_site* s2, int k);
+int acetn_hip_bond_rdm(const acetn_site* s1, const acetn_site* s2, int k, double* rho,
+                       void* workspace, size_t ws_bytes, void* stream);
 
 /* ---- building blocks, exported for the parity tests ---------------------- */
 

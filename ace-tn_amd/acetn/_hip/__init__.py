@@ -37,32 +37,35 @@ def _declare(lib):
         "acetn_hip_last_error": (ctypes.c_char_p, []),
         "acetn_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
         "acetn_hip_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
-        "acetn_hip_kernel_timing_read": (ctypes.c_int, [ctypes.POINTER(c_dbl), ctypes.POINTER(c_dbl),
+        "acetn_hip_kernel_timing_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_dbl),
+                                                        ctypes.POINTER(c_dbl),
                                                         ctypes.POINTER(c_i64)]),
         "acetn_hip_quarter_tensor_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int]),
-        "acetn_hip_quarter_tensor": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int, c_vp,
-                                                    c_vp, c_size, c_vp]),
+        "acetn_hip_quarter_tensor": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                    ctypes.c_int, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_half_projectors_workspace": (c_size, [ctypes.POINTER(AcetnSite),
                                                          ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                          c_i64, c_i64]),
         "acetn_hip_half_projectors": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
                                                      ctypes.POINTER(AcetnSite), ctypes.c_int, c_i64,
-                                                     c_i64, ctypes.c_int, c_dbl, c_u64, c_u64, c_vp,
-                                                     c_vp, ctypes.POINTER(c_i64), c_vp, c_vp, c_size,
-                                                     c_vp]),
+                                                     c_i64, ctypes.c_int, c_dbl, c_u64, c_u64,
+                                                     ctypes.c_int, c_vp, c_vp, ctypes.POINTER(c_i64),
+                                                     c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_half_projectors_begin": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
                                                            ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                            c_i64, c_i64, ctypes.c_int, c_dbl, c_u64,
-                                                           c_u64, c_vp, c_vp, c_vp, c_size, c_vp]),
+                                                           c_u64, ctypes.c_int, c_vp, c_vp, c_vp,
+                                                           c_size, c_vp]),
         "acetn_hip_half_projectors_end": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
                                                          ctypes.POINTER(AcetnSite), ctypes.c_int,
-                                                         c_i64, c_i64, ctypes.c_int, c_i64, c_vp, c_vp,
-                                                         c_vp, c_size, c_vp]),
+                                                         c_i64, c_i64, ctypes.c_int, ctypes.c_int,
+                                                         c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_renormalize_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                      c_i64, c_i64]),
         "acetn_hip_renormalize_boundary": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                           c_vp, c_vp, c_i64, c_vp, c_vp, c_i64,
-                                                          c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+                                                          ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
+                                                          c_size, c_vp]),
         "acetn_hip_site_rdm_workspace": (c_size, [ctypes.POINTER(AcetnSite)]),
         "acetn_hip_site_rdm": (ctypes.c_int, [ctypes.POINTER(AcetnSite), c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_bond_rdm_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.POINTER(AcetnSite),
@@ -74,6 +77,9 @@ def _declare(lib):
         "acetn_hip_gemm": (ctypes.c_int, [c_i64, c_i64, c_i64, ctypes.c_int, c_vp, c_i64,
                                           ctypes.c_int, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_i64,
                                           c_vp, c_size, c_vp]),
+        "acetn_hip_gemm_split": (ctypes.c_int, [c_i64, c_i64, c_i64, ctypes.c_int, c_vp, c_i64,
+                                                ctypes.c_int, c_vp, c_i64, c_dbl, c_dbl, c_vp,
+                                                c_i64, ctypes.c_int, c_vp, c_size, c_vp]),
         "acetn_hip_eigh_workspace": (c_size, [c_i64]),
         "acetn_hip_eigh": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_orthonormalize_workspace": (c_size, [c_i64, c_i64]),
@@ -196,10 +202,24 @@ def kernel_timing(enable):
     check(lib().acetn_hip_kernel_timing(int(enable)))
 
 
-def kernel_timing_read():
-    """(summed GEMM kernel ms, summed algorithmic flops, launches)."""
+# kernel classes of acetn_hip_kernel_timing_read
+K_GEMM_F64, K_GEMM_SPLIT, K_DOUBLE_LAYER = 0, 1, 2
+# precision modes (ACETN_HIP_FP64 / ACETN_HIP_MIXED)
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def precision_code(precision):
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"unknown precision {precision!r}; expected one of {sorted(PRECISIONS)}")
+
+
+def kernel_timing_read(kernel=K_GEMM_F64):
+    """(summed kernel ms, summed algorithmic flops, launches) of one kernel class."""
     ms, fl, n = c_dbl(0), c_dbl(0), c_i64(0)
-    check(lib().acetn_hip_kernel_timing_read(ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(n)))
+    check(lib().acetn_hip_kernel_timing_read(kernel, ctypes.byref(ms), ctypes.byref(fl),
+                                             ctypes.byref(n)))
     return ms.value, fl.value, n.value
 
 
@@ -210,8 +230,10 @@ def gaussian_sketch(n, q, seed, stream_id, device="cuda"):
     return out
 
 
-def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None):
-    """Dense FP64 GEMM on the matrix cores: alpha*op(A) op(B) + beta*C."""
+def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None, pieces=0):
+    """Dense GEMM on the matrix cores: alpha*op(A) op(B) + beta*C, FP64
+    (pieces=0) or split-bf16 with FP32 accumulation (pieces=2: bf16x3,
+    pieces=3: bf16x6); FP64 storage either way."""
     A = A.contiguous()
     B = B.contiguous()
     M = A.shape[1] if ta else A.shape[0]
@@ -221,6 +243,11 @@ def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None):
         C = torch.zeros(M, N, dtype=torch.float64, device=A.device)
     nb = lib().acetn_hip_gemm_workspace(M, N, K)
     ws = workspace(nb, A.device)
+    if pieces:
+        check(lib().acetn_hip_gemm_split(M, N, K, int(ta), ptr(A), A.shape[1], int(tb), ptr(B),
+                                         B.shape[1], alpha, beta, ptr(C), C.shape[1], pieces,
+                                         ptr(ws), ws.numel(), stream_handle()))
+        return C
     check(lib().acetn_hip_gemm(M, N, K, int(ta), ptr(A), A.shape[1], int(tb), ptr(B), B.shape[1],
                                alpha, beta, ptr(C), C.shape[1], ptr(ws), ws.numel(), stream_handle()))
     return C
@@ -247,7 +274,7 @@ def orthonormalize(Y):
     return Y
 
 
-def quarter_tensor(site_tensors, k):
+def quarter_tensor(site_tensors, k, precision="fp64"):
     """site_tensors: (A, [C0..C3], [E0..E3]) device tensors."""
     s, keep = make_site(*site_tensors)
     N1 = s.E_dim[k % 4][1] * s.D * s.D
@@ -255,12 +282,13 @@ def quarter_tensor(site_tensors, k):
     Q = torch.empty(N1, N2, dtype=torch.float64, device=keep[0].device)
     nb = lib().acetn_hip_quarter_tensor_workspace(ctypes.byref(s), k)
     ws = workspace(nb, Q.device)
-    check(lib().acetn_hip_quarter_tensor(ctypes.byref(s), k, ptr(Q), ptr(ws), ws.numel(),
-                                         stream_handle()))
+    check(lib().acetn_hip_quarter_tensor(ctypes.byref(s), k, precision_code(precision), ptr(Q),
+                                         ptr(ws), ws.numel(), stream_handle()))
     return Q
 
 
-def half_projectors(site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id):
+def half_projectors(site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id,
+                    precision="fp64"):
     """Returns (P1, P2, spectrum) with P1/P2 shaped (rows, cD_new)."""
     s1, keep1 = make_site(*site1)
     s4, keep4 = make_site(*site4)
@@ -274,7 +302,8 @@ def half_projectors(site1, site4, k, chi, oversampling, niter, cutoff, seed, str
                                                    oversampling)
     ws = workspace(nb, dev)
     check(lib().acetn_hip_half_projectors(ctypes.byref(s1), ctypes.byref(s4), k, chi, oversampling,
-                                          niter, cutoff, seed, stream_id, ptr(P1), ptr(P2),
+                                          niter, cutoff, seed, stream_id,
+                                          precision_code(precision), ptr(P1), ptr(P2),
                                           ctypes.byref(cD), ptr(spec), ptr(ws), ws.numel(),
                                           stream_handle()))
     n = cD.value
@@ -285,10 +314,12 @@ class ProjectorJob:
     """One half-system projector pair computed in two phases (see
     acetn_hip_half_projectors_begin/_end)."""
 
-    def __init__(self, slot, site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id):
+    def __init__(self, slot, site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id,
+                 precision="fp64"):
         self.s1, self.keep1 = make_site(*site1)
         self.s4, self.keep4 = make_site(*site4)
         self.k, self.chi, self.p, self.niter = k, chi, oversampling, niter
+        self.prec = precision_code(precision)
         self.dev = self.keep1[0].device
         self.rows = self.s1.E_dim[(3 + k) % 4][0] * self.s1.D * self.s1.D
         self.slot = slot
@@ -301,7 +332,8 @@ class ProjectorJob:
         with torch.cuda.stream(self.stream):
             check(lib().acetn_hip_half_projectors_begin(
                 ctypes.byref(self.s1), ctypes.byref(self.s4), k, chi, oversampling, niter, cutoff,
-                seed, stream_id, c_vp(self.cd_host.data_ptr()), ptr(self.spec), ptr(self.ws),
+                seed, stream_id, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec),
+                ptr(self.ws),
                 self.ws.numel(), c_vp(self.stream.cuda_stream)))
 
     def finish(self):
@@ -312,11 +344,12 @@ class ProjectorJob:
             P2 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)
             check(lib().acetn_hip_half_projectors_end(
                 ctypes.byref(self.s1), ctypes.byref(self.s4), self.k, self.chi, self.p, self.niter,
-                n, ptr(P1), ptr(P2), ptr(self.ws), self.ws.numel(), c_vp(self.stream.cuda_stream)))
+                self.prec, n, ptr(P1), ptr(P2), ptr(self.ws), self.ws.numel(),
+                c_vp(self.stream.cuda_stream)))
         return P1, P2
 
 
-def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None):
+def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None, precision="fp64"):
     """Returns (C_a, C_b, E_o): the new C[(3+k)%4], C[k], E[(3+k)%4] of s2.
     ``slot`` selects a per-slot workspace for concurrent calls."""
     s1, keep = make_site(*site1)
@@ -332,7 +365,8 @@ def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None):
     nb = lib().acetn_hip_renormalize_workspace(ctypes.byref(s1), k, cDi, cDj)
     ws = workspace(nb, dev) if slot is None else slot_workspace(slot, nb, dev)
     check(lib().acetn_hip_renormalize_boundary(ctypes.byref(s1), k, ptr(P1i), ptr(P2i), cDi,
-                                               ptr(P1j), ptr(P2j), cDj, ptr(Ca), ptr(Cb), ptr(Eo),
+                                               ptr(P1j), ptr(P2j), cDj, precision_code(precision),
+                                               ptr(Ca), ptr(Cb), ptr(Eo),
                                                ptr(ws), ws.numel(), stream_handle()))
     return Ca, Cb, Eo
 

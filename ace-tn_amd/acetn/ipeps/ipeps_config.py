@@ -2,7 +2,8 @@
 
 Same keys and defaults as the reference, plus the engine's own:
   ctmrg.backend     -- "hip" (the only engine here)
-  ctmrg.precision   -- "fp64" (plain path) | "fp32" | "bf16x3" (mixed paths)
+  ctmrg.precision   -- "fp64" (plain path) | "mixed" (split-bf16 matrix-core
+                       contractions with FP32 accumulation, FP64 QR/SVD)
   ctmrg.sketch_seed -- seed of the Philox range-finder sketch (None: drawn
                        once from torch's global generator, so
                        torch.manual_seed makes runs reproducible)
@@ -36,7 +37,7 @@ class CTMRGConfig:
     rsvd_oversampling: int = 2
     disable_progressbar: bool = False
     backend: Literal["hip"] = "hip"
-    precision: Literal["fp64", "fp32", "bf16x3"] = "fp64"
+    precision: Literal["fp64", "mixed"] = "fp64"
     sketch_seed: Optional[int] = None
     conv_tol: Optional[float] = None
 

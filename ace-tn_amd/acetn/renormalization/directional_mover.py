@@ -50,7 +50,8 @@ class DirectionalMover:
         for slot, (s1, s2, i, j, k) in enumerate(targets):
             stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
             with torch.cuda.stream(stream):
-                results.append(self._renormalize(ipeps, proj[i], proj[j], s1, k, slot))
+                results.append(self._renormalize(ipeps, proj[i], proj[j], s1, k, slot,
+                                                 self.projector_calculator.precision))
         torch.cuda.synchronize()
         for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(targets, results):
             dst = ipeps[s2]
@@ -102,15 +103,16 @@ class DirectionalMover:
     def renormalize_boundary(self, ipeps, proj1, proj2, s1, s2, i, j, k):
         """directional_mover.py:273-303: writes C[(3+k)%4], C[k], E[(3+k)%4]
         of s2."""
-        Ca, Cb, Eo = self._renormalize(ipeps, (proj1[i], proj2[i]), (proj1[j], proj2[j]), s1, k)
+        Ca, Cb, Eo = self._renormalize(ipeps, (proj1[i], proj2[i]), (proj1[j], proj2[j]), s1, k,
+                                       precision=self.projector_calculator.precision)
         dst = ipeps[s2]
         dst['C'][(3 + k) % 4] = Ca
         dst['C'][k] = Cb
         dst['E'][(3 + k) % 4] = Eo
 
     @staticmethod
-    def _renormalize(ipeps, pi, pj, s1, k, slot=None):
+    def _renormalize(ipeps, pi, pj, s1, k, slot=None, precision="fp64"):
         st = ipeps[s1]
         flat = lambda P: P.reshape(-1, P.shape[-1])
         return _hip.renormalize_boundary((st['A'], st['C'], st['E']), k, flat(pi[0]), flat(pi[1]),
-                                         flat(pj[0]), flat(pj[1]), slot=slot)
+                                         flat(pj[0]), flat(pj[1]), slot=slot, precision=precision)

@@ -28,6 +28,8 @@ class ProjectorCalculator:
         self.rsvd_niter = config.rsvd_niter
         self.rsvd_oversampling = config.rsvd_oversampling
         self.sketch = sketch if sketch is not None else SketchCounter(0)
+        self.precision = getattr(config, "precision", "fp64")
+        _hip.precision_code(self.precision)  # validates
         self.last_spectrum = None
         self.set_calculate()
 
@@ -57,7 +59,8 @@ class ProjectorCalculator:
         chi = ipeps.dims['chi']
         P1, P2, spec = _hip.half_projectors(self._site(ipeps, s1), self._site(ipeps, s4), k % 4,
                                             chi, self.rsvd_oversampling, self.rsvd_niter,
-                                            self.svd_cutoff, self.sketch.seed, self.sketch.take())
+                                            self.svd_cutoff, self.sketch.seed, self.sketch.take(),
+                                            precision=self.precision)
         self.last_spectrum = spec
         D = ipeps.dims['bond']
         rows = P1.shape[0]
@@ -83,7 +86,8 @@ class ProjectorCalculator:
             started.append(_hip.ProjectorJob(slot, self._site(ipeps, sites[0]),
                                              self._site(ipeps, sites[3]), k % 4, chi,
                                              self.rsvd_oversampling, self.rsvd_niter,
-                                             self.svd_cutoff, self.sketch.seed, self.sketch.take()))
+                                             self.svd_cutoff, self.sketch.seed, self.sketch.take(),
+                                             precision=self.precision))
         torch.cuda.synchronize()
         out = []
         for job in started:

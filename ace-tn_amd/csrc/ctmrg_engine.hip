@@ -27,6 +27,8 @@ struct Ctx {
   double* gwork = nullptr;
   size_t gwork_elems = 0;
   size_t gwork_need = 0;
+  bool mixed = false;  // ACETN_HIP_MIXED: large products on split-bf16 matrix cores
+  int split = 0;       // split level of the products enqueued now (0 = FP64)
 
   double* take(size_t n) {
     const size_t bytes = ((n * sizeof(double) + 255) / 256) * 256;
@@ -68,16 +70,28 @@ void execute(void* ws, size_t ws_bytes, void* stream, F&& body) {
   body(c);
 }
 
+// Products issued inside a SplitScope run on the split-bf16 matrix cores at
+// the given level when the context is in mixed precision, else in FP64.
+struct SplitScope {
+  Ctx& c;
+  int saved;
+  SplitScope(Ctx& ctx, int level) : c(ctx), saved(ctx.split) { c.split = c.mixed ? level : 0; }
+  ~SplitScope() { c.split = saved; }
+};
+
 void gemm(Ctx& c, const GemmDesc& g, const double* A, const double* B, double* C) {
   if (!c.run) {
-    c.gwork_need = std::max(c.gwork_need, gemm_f64_work_elems(g, 1));
+    c.gwork_need = std::max({c.gwork_need, gemm_f64_work_elems(g, 1), gemm_split_work_elems(g, 1)});
     return;
   }
   if (!c.launch) return;
   GemmBatch b{};
   b.count = 1;
   b.A[0] = A; b.B[0] = B; b.C[0] = C;
-  gemm_f64(g, b, c.gwork, c.gwork_elems, c.st);
+  if (c.split > 0)
+    gemm_split(g, b, c.split, DType::F64, DType::F64, DType::F64, c.gwork, c.gwork_elems, c.st);
+  else
+    gemm_f64(g, b, c.gwork, c.gwork_elems, c.st);
 }
 
 // dense row-major helpers
@@ -131,6 +145,7 @@ QuarterDims quarter_dims(const acetn_site* s, int k) {
 }
 
 void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
+  SplitScope sc(c, 3);
   const QuarterDims qd = quarter_dims(s, k);
   const int kk = k & 3;
   const int64_t D = qd.D, D2 = D * D;
@@ -170,26 +185,33 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
 // Result is written back into Y.
 // ---------------------------------------------------------------------------
 void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
+  SplitScope sc(c, 0);
   double* G = c.take((size_t)q * q);
   double* Rinv = c.take((size_t)q * q);
   double* Yt = c.take((size_t)m * q);
-  int* dead = c.take_int((size_t)q + 2);
+  int* dead = c.take_int((size_t)q + 2);  // dead[0..q-1] flags, dead[q] = any dead
   const size_t pb = (size_t)q * (q + 1) / 2;
   double* scr = (pb * 8 > 150 * 1024) ? c.take(pb) : nullptr;
   const double u = 1.1102230246251565e-16;
-  // pass 0: shifted; pass 1: plain with rank guard (dead columns -> refilled);
-  // passes 2, 3: plain, orthogonalising the refilled columns.
+  // pass 0: shifted (Y -> Yt); pass 1: plain with rank guard (Yt -> Y), dead
+  // columns refilled; passes 2, 3 (Y -> Yt -> Y) orthogonalise the refilled
+  // columns and are skipped on the device when no column died.
   for (int pass = 0; pass < 4; ++pass) {
     double* src = (pass % 2 == 0) ? Y : Yt;
     double* dst = (pass % 2 == 0) ? Yt : Y;
-    mm(c, (int)q, (int)q, (int)m, true, src, q, false, src, q, G, q);
+    const int* run_if = (pass >= 2 && c.launch) ? dead + q : nullptr;
+    GemmDesc gg = dense_desc((int)q, (int)q, (int)m, true, q, false, q, q);
+    gg.run_if = run_if;
+    gemm(c, gg, src, src, G);
     const double shift = (pass == 0) ? 11.0 * ((double)m * q + (double)q * (q + 1)) * u : 0.0;
     if (c.launch) chol_rinv(G, Rinv, (int)q, shift, pass == 0 ? 0.0 : 1e-14, c.st, scr,
-                         pass == 1 ? dead : nullptr, pass >= 2 ? dead : nullptr);
-    mm(c, (int)m, (int)q, (int)q, false, src, q, false, Rinv, q, dst, q);
+                            pass == 1 ? dead : nullptr, pass >= 2 ? dead : nullptr);
+    GemmDesc gr = dense_desc((int)m, (int)q, (int)q, false, q, false, q, q);
+    gr.run_if = run_if;
+    gemm(c, gr, src, Rinv, dst);
     if (pass == 1 && c.launch) refill_dead_columns(dst, m, (int)q, dead, 0x243F6A8885A308D3ull, c.st);
   }
-  // four passes leave the result in Y
+  // the result is in Y whether or not passes 2, 3 ran
 }
 
 // ---------------------------------------------------------------------------
@@ -199,6 +221,7 @@ void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
 // Outputs: w (q, descending eigenvalues of B B^T), UB (q x q).
 // ---------------------------------------------------------------------------
 void small_svd(Ctx& c, const double* BT, int64_t n, int64_t q, double* w, double* UB) {
+  SplitScope sc(c, 0);
   double* G = c.take((size_t)q * q);
   double* W1 = c.take((size_t)q * q);
   double* W2 = c.take((size_t)q * q);
@@ -259,16 +282,21 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   quarter_tensor(c, s1, k, Q1);
   quarter_tensor(c, s4, k + 3, Q4);
   if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st);
+  // mixed precision: the sketch and the first power iteration only steer the
+  // range finder (bf16x3); later products carry the basis itself (bf16x6)
+  SplitScope sc(c, 2);
   // Y = Q1 (Q4 Omega)
   mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Om, q, W, q);
   mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);
   for (int it = 0; it < niter; ++it) {
+    if (it >= 1) c.split = c.mixed ? 3 : 0;
     orthonormalize(c, Y, m, q);
     mm(c, (int)kk, (int)q, (int)m, true, Q1, kk, false, Y, q, W, q);   // Q1^H Y
     mm(c, (int)n, (int)q, (int)kk, true, Q4, n, false, W, q, Z, q);    // Q4^H (.)
     mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Z, q, W, q);   // Q4 (.)
     mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);  // Q1 (.)
   }
+  c.split = c.mixed ? 3 : 0;
   orthonormalize(c, Y, m, q);
   // B^T = Q4^H (Q1^H Qm): n x q ; keep Wq = Q1^H Qm for P1
   mm(c, (int)kk, (int)q, (int)m, true, Q1, kk, false, Y, q, Wq, q);
@@ -277,9 +305,11 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   if (!c.run) {
     // size the split-K scratch of the post-truncation products at cD = chi
     const int cDp = (int)std::min<int64_t>(chi, q);
-    mm(c, (int)kk, cDp, (int)q, false, nullptr, q, false, nullptr, cDp, nullptr, cDp);
-    mm(c, (int)n, cDp, (int)q, false, nullptr, q, false, nullptr, cDp, nullptr, cDp);
-    mm(c, (int)kk, cDp, (int)n, false, nullptr, n, false, nullptr, cDp, nullptr, cDp);
+    for (int cd = 1; cd <= cDp; cd = (cd == cDp) ? cDp + 1 : std::min(2 * cd, cDp)) {
+      mm(c, (int)kk, cd, (int)q, false, nullptr, q, false, nullptr, cd, nullptr, cd);
+      mm(c, (int)n, cd, (int)q, false, nullptr, q, false, nullptr, cd, nullptr, cd);
+      mm(c, (int)kk, cd, (int)n, false, nullptr, n, false, nullptr, cd, nullptr, cd);
+    }
     return;
   }
   int cD = 0;
@@ -299,11 +329,13 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   }
   ACETN_REQUIRE(cD >= 1 && cD <= std::min<int64_t>(chi, q), "half system: no singular value above the cutoff");
   // P1 = Q1^H U S_n^-1/2 = Wq (UB S_n^-1/2)
+  c.split = 0;
   scale_columns(UB, (int)q, s, cD, 0, X, c.st);
   mm(c, (int)kk, cD, (int)q, false, Wq, q, false, X, cD, out.P1, cD);
   // P2 = Q4 V S_n^-1/2,  V = B^T UB S^-1
   scale_columns(UB, (int)q, s, cD, 1, X, c.st);
   mm(c, (int)n, cD, (int)q, false, BT, q, false, X, cD, Vp, cD);
+  c.split = c.mixed ? 3 : 0;
   mm(c, (int)kk, cD, (int)n, false, Q4, n, false, Vp, cD, out.P2, cD);
   fix_projector_gauge(out.P1, out.P2, kk, kk, cD, c.st);
 }
@@ -314,6 +346,7 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
 void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const double* P2i,
                  int64_t cDi, const double* P1j, const double* P2j, int64_t cDj, double* Ca,
                  double* Cb, double* Eo) {
+  SplitScope sc(c, 3);
   k &= 3;
   const int64_t D = s1->D, D2 = D * D;
   double* nrm = c.take(1024);
@@ -569,6 +602,10 @@ void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double*
        r1, r2, rho);
 }
 
+void check_precision(int p) {
+  ACETN_REQUIRE(p == ACETN_HIP_FP64 || p == ACETN_HIP_MIXED, "unknown precision mode");
+}
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -599,8 +636,9 @@ int acetn_hip_kernel_timing(int enable) {
   return guarded([&] { kernel_timing_enable(enable != 0); });
 }
 
-int acetn_hip_kernel_timing_read(double* host_ms, double* host_flops, int64_t* host_launches) {
-  return guarded([&] { kernel_timing_read(host_ms, host_flops, host_launches); });
+int acetn_hip_kernel_timing_read(int kernel, double* host_ms, double* host_flops,
+                                 int64_t* host_launches) {
+  return guarded([&] { kernel_timing_read(kernel, host_ms, host_flops, host_launches); });
 }
 
 int acetn_hip_device_info(char* buf, size_t len) {
@@ -620,11 +658,15 @@ size_t acetn_hip_quarter_tensor_workspace(const acetn_site* s, int k) {
   return r;
 }
 
-int acetn_hip_quarter_tensor(const acetn_site* s, int k, double* Q, void* ws, size_t ws_bytes,
-                             void* stream) {
+int acetn_hip_quarter_tensor(const acetn_site* s, int k, int precision, double* Q, void* ws,
+                             size_t ws_bytes, void* stream) {
   return guarded([&] {
     check_site(s);
-    execute(ws, ws_bytes, stream, [&](Ctx& c) { quarter_tensor(c, s, k, Q); });
+    check_precision(precision);
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
+      quarter_tensor(c, s, k, Q);
+    });
   });
 }
 
@@ -644,16 +686,18 @@ size_t acetn_hip_half_projectors_workspace(const acetn_site* s1, const acetn_sit
 
 int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
                               int64_t oversampling, int niter, double svd_cutoff, uint64_t seed,
-                              uint64_t sketch_stream, double* P1, double* P2,
+                              uint64_t sketch_stream, int precision, double* P1, double* P2,
                               int64_t* host_cD_new, double* spectrum, void* ws, size_t ws_bytes,
                               void* stream) {
   return guarded([&] {
     check_site(s1);
     check_site(s4);
+    check_precision(precision);
     ACETN_REQUIRE(P1 && P2 && host_cD_new, "half_projectors: null output");
     ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
     HalfOut o{P1, P2, host_cD_new, nullptr, 0, spectrum};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
       half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o);
     });
   });
@@ -661,29 +705,34 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
 
 int acetn_hip_half_projectors_begin(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
                                     int64_t oversampling, int niter, double svd_cutoff,
-                                    uint64_t seed, uint64_t sketch_stream, int32_t* pinned_cD_new,
-                                    double* spectrum, void* ws, size_t ws_bytes, void* stream) {
+                                    uint64_t seed, uint64_t sketch_stream, int precision,
+                                    int32_t* pinned_cD_new, double* spectrum, void* ws,
+                                    size_t ws_bytes, void* stream) {
   return guarded([&] {
     check_site(s1);
     check_site(s4);
+    check_precision(precision);
     ACETN_REQUIRE(pinned_cD_new, "half_projectors_begin: null cD target");
     ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
     HalfOut o{nullptr, nullptr, nullptr, pinned_cD_new, 0, spectrum};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
       half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o, 1);
     });
   });
 }
 
 int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
-                                  int64_t oversampling, int niter, int64_t cD_new, double* P1,
-                                  double* P2, void* ws, size_t ws_bytes, void* stream) {
+                                  int64_t oversampling, int niter, int precision, int64_t cD_new,
+                                  double* P1, double* P2, void* ws, size_t ws_bytes, void* stream) {
   return guarded([&] {
     check_site(s1);
     check_site(s4);
+    check_precision(precision);
     ACETN_REQUIRE(P1 && P2, "half_projectors_end: null output");
     HalfOut o{P1, P2, nullptr, nullptr, (int)cD_new, nullptr};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
       half_projectors(c, s1, s4, k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
     });
   });
@@ -703,13 +752,16 @@ size_t acetn_hip_renormalize_workspace(const acetn_site* s1, int k, int64_t cD_i
 
 int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1_i,
                                    const double* P2_i, int64_t cD_i, const double* P1_j,
-                                   const double* P2_j, int64_t cD_j, double* C_a, double* C_b,
-                                   double* E_o, void* ws, size_t ws_bytes, void* stream) {
+                                   const double* P2_j, int64_t cD_j, int precision, double* C_a,
+                                   double* C_b, double* E_o, void* ws, size_t ws_bytes,
+                                   void* stream) {
   return guarded([&] {
     check_site(s1);
+    check_precision(precision);
     ACETN_REQUIRE(cD_i >= 1 && cD_j >= 1, "renormalize: empty projector");
     ACETN_REQUIRE(P1_i && P2_i && P1_j && P2_j && C_a && C_b && E_o, "renormalize: null pointer");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
       renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o);
     });
   });
@@ -775,6 +827,22 @@ int acetn_hip_gemm(int64_t M, int64_t N, int64_t K, int ta, const double* A, int
                       K < (1LL << 31), "gemm: bad shape");
     ACETN_REQUIRE(A && B && C, "gemm: null pointer");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      mm(c, (int)M, (int)N, (int)K, ta != 0, A, lda, tb != 0, B, ldb, C, ldc, alpha, beta);
+    });
+  });
+}
+
+int acetn_hip_gemm_split(int64_t M, int64_t N, int64_t K, int ta, const double* A, int64_t lda,
+                         int tb, const double* B, int64_t ldb, double alpha, double beta, double* C,
+                         int64_t ldc, int pieces, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    ACETN_REQUIRE(M >= 0 && N >= 0 && K >= 0 && M < (1LL << 31) && N < (1LL << 31) &&
+                      K < (1LL << 31), "gemm_split: bad shape");
+    ACETN_REQUIRE(A && B && C, "gemm_split: null pointer");
+    ACETN_REQUIRE(pieces == 2 || pieces == 3, "gemm_split: pieces must be 2 or 3");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = true;
+      c.split = pieces;
       mm(c, (int)M, (int)N, (int)K, ta != 0, A, lda, tb != 0, B, ldb, C, ldc, alpha, beta);
     });
   });

@@ -173,8 +173,12 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
   const int64_t nblk = (int64_t)g.n1 * g.n2;
   const int64_t ngroups = (nblk + S::NB - 1) / S::NB;
   const int grid = (int)std::min<int64_t>(ngroups, 4096);
+  const double d6 = (double)S::D2 * S::D2 * S::D2;
+  const double flops = (double)nblk * (2.0 * g.d * d6 + 2.0 * (g.open_p ? g.d * g.d : g.d) * d6);
+  hipEvent_t e0 = kernel_timing_begin(s);
   hipLaunchKernelGGL(dl_mfma_kernel<D>, dim3(grid), dim3(NT), lds, s, g, X, A, Y);
   ACETN_HIP_CHECK(hipGetLastError());
+  kernel_timing_end(e0, s, kKDoubleLayer, flops);
 }
 }  // namespace
 

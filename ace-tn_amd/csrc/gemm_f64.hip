@@ -21,15 +21,17 @@ namespace acetn {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// ---- optional live timing of every gemm_f64_kernel launch (HIP events on
-// the launch stream), used by the benchmark for the roofline figure.
+// ---- optional live timing of the matrix-core kernel launches (HIP events on
+// the launch stream), per kernel class, used by the benchmark for the
+// roofline figures.
 namespace {
 struct KTimer {
   std::mutex mu;
   bool on = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-  double flops = 0.0;
-  int64_t launches = 0;
+  struct Rec { hipEvent_t e0, e1; int cls; };
+  std::vector<Rec> ev;
+  double flops[kKClasses] = {};
+  int64_t launches[kKClasses] = {};
 };
 KTimer& ktimer() {
   static KTimer t;
@@ -40,26 +42,48 @@ KTimer& ktimer() {
 void kernel_timing_enable(bool on) {
   KTimer& t = ktimer();
   std::lock_guard<std::mutex> g(t.mu);
-  for (auto& e : t.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  for (auto& e : t.ev) { (void)hipEventDestroy(e.e0); (void)hipEventDestroy(e.e1); }
   t.ev.clear();
-  t.flops = 0.0;
-  t.launches = 0;
+  for (int c = 0; c < kKClasses; ++c) { t.flops[c] = 0.0; t.launches[c] = 0; }
   t.on = on;
 }
 
-void kernel_timing_read(double* ms, double* flops, int64_t* launches) {
+void kernel_timing_read(int cls, double* ms, double* flops, int64_t* launches) {
+  ACETN_REQUIRE(cls >= 0 && cls < kKClasses, "kernel_timing_read: unknown kernel class");
   KTimer& t = ktimer();
   std::lock_guard<std::mutex> g(t.mu);
   double tot = 0.0;
   for (auto& e : t.ev) {
-    ACETN_HIP_CHECK(hipEventSynchronize(e.second));
+    if (e.cls != cls) continue;
+    ACETN_HIP_CHECK(hipEventSynchronize(e.e1));
     float x = 0.f;
-    ACETN_HIP_CHECK(hipEventElapsedTime(&x, e.first, e.second));
+    ACETN_HIP_CHECK(hipEventElapsedTime(&x, e.e0, e.e1));
     tot += x;
   }
   *ms = tot;
-  *flops = t.flops;
-  *launches = t.launches;
+  *flops = t.flops[cls];
+  *launches = t.launches[cls];
+}
+
+hipEvent_t kernel_timing_begin(hipStream_t s) {
+  KTimer& t = ktimer();
+  if (!t.on) return nullptr;
+  hipEvent_t e0 = nullptr;
+  ACETN_HIP_CHECK(hipEventCreate(&e0));
+  ACETN_HIP_CHECK(hipEventRecord(e0, s));
+  return e0;
+}
+
+void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops) {
+  if (e0 == nullptr) return;
+  KTimer& t = ktimer();
+  hipEvent_t e1 = nullptr;
+  ACETN_HIP_CHECK(hipEventCreate(&e1));
+  ACETN_HIP_CHECK(hipEventRecord(e1, s));
+  std::lock_guard<std::mutex> lg(t.mu);
+  t.ev.push_back({e0, e1, cls});
+  t.flops[cls] += flops;
+  t.launches[cls] += 1;
 }
 
 namespace {
@@ -78,6 +102,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
   constexpr int AE = BM * BK / NT, BE = BN * BK / NT;  // elements per thread
   __shared__ double As[2][BK][LA];
   __shared__ double Bs[2][BK][LB];
+  if (g.run_if && *g.run_if == 0) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -196,6 +221,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
 
 __global__ void splitk_reduce_kernel(GemmDesc g, GemmBatch bt, int ksplit,
                                      const double* __restrict__ work) {
+  if (g.run_if && *g.run_if == 0) return;
   const int64_t mn = (int64_t)g.M * g.N;
   const int prob = blockIdx.y;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
@@ -256,25 +282,13 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
   int ks = choose_ksplit(g, b.count);
   if (ks > 1 && (work == nullptr || work_elems < (size_t)ks * b.count * g.M * g.N)) ks = 1;
   const Tile t = choose_tile(g);
-  KTimer& kt = ktimer();
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (kt.on) {
-    ACETN_HIP_CHECK(hipEventCreate(&e0));
-    ACETN_HIP_CHECK(hipEventCreate(&e1));
-    ACETN_HIP_CHECK(hipEventRecord(e0, s));
-  }
+  hipEvent_t e0 = kernel_timing_begin(s);
   if (t.wm == 2 && t.wn == 5) launch_gemm<2, 5>(g, b, ks, work, s);
   else if (t.wm == 1 && t.wn == 5) launch_gemm<1, 5>(g, b, ks, work, s);
   else if (t.wm == 4 && t.wn == 4) launch_gemm<4, 4>(g, b, ks, work, s);
   else launch_gemm<2, 2>(g, b, ks, work, s);
   ACETN_HIP_CHECK(hipGetLastError());
-  if (kt.on) {
-    ACETN_HIP_CHECK(hipEventRecord(e1, s));
-    std::lock_guard<std::mutex> lg(kt.mu);
-    kt.ev.emplace_back(e0, e1);
-    kt.flops += 2.0 * g.M * g.N * (double)g.K * b.count;
-    kt.launches += 1;
-  }
+  kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count);
   if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);

@@ -80,6 +80,7 @@ struct GemmDesc {
   IdxMap bk, bn;  // B(k,n) = B[bk(k) + bn(n)]
   IdxMap cm, cn;  // C(m,n) = C[cm(m) + cn(n)]
   double alpha, beta;
+  const int* run_if = nullptr;  // device flag: when given and zero the product is skipped
 };
 
 struct GemmBatch {
@@ -95,9 +96,25 @@ struct GemmBatch {
 void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_elems,
               hipStream_t s);
 size_t gemm_f64_work_elems(const GemmDesc& g, int count);
-// Live timing of gemm_f64_kernel launches (events on the launch stream).
+
+// Split-bf16 GEMM (v_mfma_f32_16x16x32_bf16, FP32 accumulation): operands
+// are rounded to FP32 and split into ``split`` bf16 pieces (2: "bf16x3",
+// 3: "bf16x6" ~ FP32 accuracy).  Operand / result element types per DType;
+// the GemmBatch pointers are reinterpreted accordingly.
+enum class DType : int { F64 = 0, F32 = 1 };
+void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DType tb, DType tc,
+                double* work, size_t work_elems, hipStream_t s);
+size_t gemm_split_work_elems(const GemmDesc& g, int count);
+
+// Live timing of the matrix-core kernels (HIP events on the launch stream),
+// per kernel class; the benchmark's roofline figures come from here.
+enum KernelClass : int { kKGemmF64 = 0, kKGemmSplit = 1, kKDoubleLayer = 2, kKClasses = 3 };
 void kernel_timing_enable(bool on);
-void kernel_timing_read(double* ms, double* flops, int64_t* launches);
+void kernel_timing_read(int cls, double* ms, double* flops, int64_t* launches);
+// Returns an event recorded on ``s`` (nullptr when timing is off); pass it to
+// kernel_timing_end after the timed launch.
+hipEvent_t kernel_timing_begin(hipStream_t s);
+void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops);
 
 // Convenience: dense row-major C = alpha*op(A) op(B) + beta*C.
 // A is (M x K) if !ta else stored (K x M); lda/ldb/ldc are row strides.

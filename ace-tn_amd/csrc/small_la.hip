@@ -16,7 +16,7 @@
 namespace acetn {
 namespace {
 
-constexpr int LA_NT = 256;     // Cholesky: latency-bound steps, cheap barriers
+constexpr int LA_NT = 64;      // Cholesky: one wave, so the per-step barriers cost nothing
 constexpr int JAC_NT = 1024;   // Jacobi: LDS-latency-bound, needs many waves
 constexpr size_t kLdsBudget = 150 * 1024;
 
@@ -133,8 +133,15 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
     }
     __syncthreads();
   }
-  if (dead_out)
-    for (int i = tid; i < q; i += LA_NT) dead_out[i] = dead[i];
+  if (dead_out) {
+    int anyd = 0;
+    for (int i = tid; i < q; i += LA_NT) {
+      dead_out[i] = dead[i];
+      anyd |= dead[i];
+    }
+    anyd = __syncthreads_or(anyd);
+    if (tid == 0) dead_out[q] = anyd;
+  }
 
   // in-place inverse of the packed upper factor, column by column:
   // Rinv[0:j, j] = -Rinv[0:j, 0:j] R[0:j, j] / R[j, j].  Dead columns/rows are 0.
@@ -214,7 +221,6 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
   double* S = use_lds ? lds : gscratch;
   __shared__ double cs_c[256], cs_s[256];
   __shared__ short pp[256], qq_[256];
-  __shared__ int any_rot;
   const int tid = threadIdx.x;
   const int n = q + (q & 1);
   const int half = n / 2;
@@ -242,9 +248,7 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
   }
   int sweep = 0;
   for (; sweep < kMaxSweeps; ++sweep) {
-    __syncthreads();  // every thread has read the previous sweep's flag
-    if (tid == 0) any_rot = 0;
-    __syncthreads();
+    int my_rot = 0;
     for (int r = 0; r < n - 1; ++r) {
       // rotation parameters (and this round's pair table)
       for (int a = tid; a < half; a += JAC_NT) {
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
             s = t * c;
-            any_rot = 1;
+            my_rot = 1;
           }
         }
         cs_c[a] = c; cs_s[a] = s;
@@ -298,8 +302,7 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
       }
       __syncthreads();
     }
-    const int rotated = any_rot;  // read after the round loop's final barrier
-    if (!rotated) { ++sweep; break; }
+    if (!__syncthreads_or(my_rot)) { ++sweep; break; }
   }
   if (tid == 0) info[0] = sweep;
   for (int i = tid; i < q; i += JAC_NT) w[i] = S[pk(i, i)];
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
 // Replay the recorded rotations on the rows of V (V starts as identity).
 // Each workgroup owns ``rows`` rows in LDS.  Afterwards the eigenpairs are
 // sorted by descending eigenvalue (ties: lower index first).
-constexpr int kReplayRounds = 16;
+constexpr int kReplayRounds = 32;
 
 __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __restrict__ rot,
                                                              const int* __restrict__ info, int q,

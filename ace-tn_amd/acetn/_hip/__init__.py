@@ -80,6 +80,39 @@ def _declare(lib):
         "acetn_hip_gemm_split": (ctypes.c_int, [c_i64, c_i64, c_i64, ctypes.c_int, c_vp, c_i64,
                                                 ctypes.c_int, c_vp, c_i64, c_dbl, c_dbl, c_vp,
                                                 c_i64, ctypes.c_int, c_vp, c_size, c_vp]),
+        "acetn_hip_bond_decompose_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_bond_decompose": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_i64, c_i64, c_vp,
+                                                    c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_norm_tensor_workspace": (c_size, [ctypes.POINTER(AcetnSite),
+                                                     ctypes.POINTER(AcetnSite), ctypes.c_int]),
+        "acetn_hip_norm_tensor": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
+                                                 ctypes.POINTER(AcetnSite), ctypes.c_int, c_vp,
+                                                 c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_apply_gate_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_apply_gate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_size,
+                                                c_vp]),
+        "acetn_hip_positive_approx_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_positive_approx": (ctypes.c_int, [c_vp, c_i64, c_i64, c_dbl, c_vp,
+                                                     ctypes.POINTER(ctypes.c_int32), c_vp, c_size,
+                                                     c_vp]),
+        "acetn_hip_gauge_fix_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_gauge_fix": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp,
+                                               c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_als_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_als_initialize": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_size,
+                                                    c_vp]),
+        "acetn_hip_als_solve": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, ctypes.c_int,
+                                               c_dbl, ctypes.c_int, c_dbl, ctypes.POINTER(c_i64),
+                                               ctypes.POINTER(c_dbl),
+                                               ctypes.POINTER(ctypes.c_int32), c_vp, c_size,
+                                               c_vp]),
+        "acetn_hip_finalize_bond_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_finalize_bond": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
+                                                   c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_simple_update_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_simple_update_bond": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp * 4,
+                                                        c_vp * 4, c_vp, c_i64, c_i64, c_dbl, c_vp,
+                                                        c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_eigh_workspace": (c_size, [c_i64]),
         "acetn_hip_eigh": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_orthonormalize_workspace": (c_size, [c_i64, c_i64]),
@@ -394,3 +427,139 @@ def bond_rdm(site1, site2, k):
     check(lib().acetn_hip_bond_rdm(ctypes.byref(s1), ctypes.byref(s2), k, ptr(rho), ptr(ws),
                                    ws.numel(), stream_handle()))
     return rho
+
+
+# ------------------------------------------------------------ full update
+ALS_METHODS = {"cholesky": 0, "pinv": 1}
+
+
+def _f64(*shape, device):
+    return torch.empty(*shape, dtype=torch.float64, device=device)
+
+
+def bond_decompose(A1, A2, k):
+    """(a1q, a1r, a2q, a2r) of the bond frame k (tensor_update.py:52-68)."""
+    D, d = A1.shape[0], A1.shape[4]
+    nD = D * d
+    dev = A1.device
+    A1, A2 = A1.contiguous(), A2.contiguous()
+    a1q, a2q = _f64(D, D, D, nD, device=dev), _f64(D, D, D, nD, device=dev)
+    a1r, a2r = _f64(nD, D, d, device=dev), _f64(nD, D, d, device=dev)
+    ws = workspace(lib().acetn_hip_bond_decompose_workspace(D, d), dev)
+    check(lib().acetn_hip_bond_decompose(ptr(A1), ptr(A2), k, D, d, ptr(a1q), ptr(a1r), ptr(a2q),
+                                         ptr(a2r), ptr(ws), ws.numel(), stream_handle()))
+    return a1q, a1r, a2q, a2r
+
+
+def norm_tensor(site1, site2, k, a1q, a2q):
+    """n12[y][x][Y][X] (full_update.py:163-227); site = (A, C, E)."""
+    s1, keep1 = make_site(*site1)
+    s2, keep2 = make_site(*site2)
+    nD = a1q.shape[3]
+    dev = a1q.device
+    n12 = _f64(nD, nD, nD, nD, device=dev)
+    ws = workspace(lib().acetn_hip_norm_tensor_workspace(ctypes.byref(s1), ctypes.byref(s2), k), dev)
+    check(lib().acetn_hip_norm_tensor(ctypes.byref(s1), ctypes.byref(s2), k, ptr(a1q.contiguous()),
+                                      ptr(a2q.contiguous()), ptr(n12), ptr(ws), ws.numel(),
+                                      stream_handle()))
+    return n12
+
+
+def apply_gate(a1r, a2r, gate):
+    nD, D, d = a1r.shape
+    a12g = _f64(nD, nD, d, d, device=a1r.device)
+    ws = workspace(lib().acetn_hip_apply_gate_workspace(D, d), a1r.device)
+    check(lib().acetn_hip_apply_gate(ptr(a1r.contiguous()), ptr(a2r.contiguous()),
+                                     ptr(gate.contiguous()), D, d, ptr(a12g), ptr(ws), ws.numel(),
+                                     stream_handle()))
+    return a12g
+
+
+def positive_approx(n12, D, d, cutoff):
+    """nz (nD, nD, nD^2) and the number of diagonal shifts applied."""
+    nD = n12.shape[0]
+    nz = _f64(nD, nD, nD * nD, device=n12.device)
+    shifts = ctypes.c_int32(0)
+    ws = workspace(lib().acetn_hip_positive_approx_workspace(D, d), n12.device)
+    check(lib().acetn_hip_positive_approx(ptr(n12.contiguous()), D, d, cutoff, ptr(nz),
+                                          ctypes.byref(shifts), ptr(ws), ws.numel(),
+                                          stream_handle()))
+    return nz, shifts.value
+
+
+def gauge_fix(nz, a12g, D, d, atol):
+    """(n12, a12g, nzxr_inv, nzyr_inv) of full_update.py:300-343."""
+    nD = nz.shape[0]
+    dev = nz.device
+    n12 = _f64(nD, nD, nD, nD, device=dev)
+    a12g_out = _f64(nD, nD, d, d, device=dev)
+    nzxr_inv, nzyr_inv = _f64(nD, nD, device=dev), _f64(nD, nD, device=dev)
+    ws = workspace(lib().acetn_hip_gauge_fix_workspace(D, d), dev)
+    check(lib().acetn_hip_gauge_fix(ptr(nz.contiguous()), ptr(a12g.contiguous()), D, d, atol,
+                                    ptr(n12), ptr(a12g_out), ptr(nzxr_inv), ptr(nzyr_inv), ptr(ws),
+                                    ws.numel(), stream_handle()))
+    return n12, a12g_out, nzxr_inv, nzyr_inv
+
+
+def als_initialize(a12g, D):
+    """The plain ALS start (als_solver.py:117-146): (a1r, a2r)."""
+    nD, _, d, _ = a12g.shape
+    dev = a12g.device
+    a1r, a2r = _f64(nD, D, d, device=dev), _f64(nD, D, d, device=dev)
+    ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)
+    check(lib().acetn_hip_als_initialize(ptr(a12g.contiguous()), D, d, ptr(a1r), ptr(a2r), ptr(ws),
+                                         ws.numel(), stream_handle()))
+    return a1r, a2r
+
+
+def als_solve(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
+    """Device ALS from the start (a1r, a2r); returns (a1r, a2r, sweeps, cost, flags)."""
+    nD, D, d = a1r.shape
+    dev = a1r.device
+    if method not in ALS_METHODS:
+        raise ValueError(f"Unknown ALS method: {method}")
+    if method != "cholesky":
+        raise NotImplementedError("als_method='pinv' is not provided by the HIP engine; "
+                                  "use 'cholesky'")
+    b1, b2 = a1r.clone().contiguous(), a2r.clone().contiguous()
+    sweeps, cost, info = c_i64(0), c_dbl(0.0), ctypes.c_int32(0)
+    ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)
+    check(lib().acetn_hip_als_solve(ptr(n12.contiguous()), ptr(a12g.contiguous()), ptr(b1),
+                                    ptr(b2), D, d, int(niter), float(tol), ALS_METHODS[method],
+                                    float(epsilon), ctypes.byref(sweeps), ctypes.byref(cost),
+                                    ctypes.byref(info), ptr(ws), ws.numel(), stream_handle()))
+    return b1, b2, sweeps.value, cost.value, info.value
+
+
+def finalize_bond(a1r, a2r, nzyr_inv, nzxr_inv, a1q, a2q, k):
+    """Updated site tensors (site frame, unit norm) of full_update.py:123-161
+    + tensor_update.py:26-38,70-75."""
+    nD, D, d = a1r.shape
+    dev = a1r.device
+    A1, A2 = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
+    ws = workspace(lib().acetn_hip_finalize_bond_workspace(D, d), dev)
+    gy = ptr(nzyr_inv.contiguous()) if nzyr_inv is not None else None
+    gx = ptr(nzxr_inv.contiguous()) if nzxr_inv is not None else None
+    check(lib().acetn_hip_finalize_bond(ptr(a1r.contiguous()), ptr(a2r.contiguous()), gy, gx,
+                                        ptr(a1q.contiguous()), ptr(a2q.contiguous()), k, D, d,
+                                        ptr(A1), ptr(A2), ptr(ws), ws.numel(), stream_handle()))
+    return A1, A2
+
+
+def simple_update_bond(A1, A2, k, lam1, lam2, gate, cutoff=1e-12):
+    """(A1', A2', lambda') of simple_update.py:28-76; lam1/lam2 are four
+    device D-vectors (None for the bond's own leg)."""
+    D, d = A1.shape[0], A1.shape[4]
+    dev = A1.device
+    A1o, A2o = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
+    lam = _f64(D, device=dev)
+    keep = [t.contiguous() for t in list(lam1) + list(lam2) if t is not None]
+    l1 = (c_vp * 4)(*[ptr(t.contiguous()) if t is not None else None for t in lam1])
+    l2 = (c_vp * 4)(*[ptr(t.contiguous()) if t is not None else None for t in lam2])
+    ws = workspace(lib().acetn_hip_simple_update_workspace(D, d), dev)
+    check(lib().acetn_hip_simple_update_bond(ptr(A1.contiguous()), ptr(A2.contiguous()), k, l1, l2,
+                                             ptr(gate.contiguous()), D, d, cutoff, ptr(A1o),
+                                             ptr(A2o), ptr(lam), ptr(ws), ws.numel(),
+                                             stream_handle()))
+    del keep
+    return A1o, A2o, lam

@@ -13,7 +13,8 @@ class Model(ABC):
 
     @staticmethod
     def bond_direction(bond):
-        return {0: '-x', 1: '+y', 2: '+x', 3: '-y'}.get(bond[2], None)
+        _, _, k = bond
+        return {0: '-x', 1: '+y', 2: '+x', 3: '-y'}.get(k, None)
 
     def initial_site_state(self, site):
         return [1, ] + [0, ] * (self.dim - 1)

@@ -5,8 +5,7 @@
 // pass in "measure" mode only sizes the scratch buffers (the *_workspace
 // queries), the second pass carves them out of the caller's workspace and
 // enqueues the kernels on the caller's stream.
-#include "internal.h"
-#include "../../include/acetn_hip.h"
+#include "engine.h"
 
 #include <algorithm>
 #include <cmath>
@@ -15,108 +14,6 @@
 
 namespace acetn {
 namespace {
-
-thread_local std::string g_last_error;
-
-struct Ctx {
-  char* base = nullptr;
-  size_t cap = 0, off = 0;
-  bool run = false;     // buffers are carved out of the workspace
-  bool launch = false;  // kernels are enqueued (false while replaying an earlier phase)
-  hipStream_t st = nullptr;
-  double* gwork = nullptr;
-  size_t gwork_elems = 0;
-  size_t gwork_need = 0;
-  bool mixed = false;  // ACETN_HIP_MIXED: large products on split-bf16 matrix cores
-  int split = 0;       // split level of the products enqueued now (0 = FP64)
-
-  double* take(size_t n) {
-    const size_t bytes = ((n * sizeof(double) + 255) / 256) * 256;
-    if (!run) { off += bytes; return nullptr; }
-    ACETN_REQUIRE(off + bytes <= cap, "workspace too small");
-    double* p = reinterpret_cast<double*>(base + off);
-    off += bytes;
-    return p;
-  }
-  int* take_int(size_t n) { return reinterpret_cast<int*>(take((n + 1) / 2)); }
-};
-
-// Run ``body`` in measure mode and return the bytes it needs (gemm split-K
-// scratch first, then the carved buffers).
-template <class F>
-size_t measure(F&& body) {
-  Ctx c;
-  body(c);
-  return c.off + ((c.gwork_need * sizeof(double) + 255) / 256) * 256 + 256;
-}
-
-template <class F>
-void execute(void* ws, size_t ws_bytes, void* stream, F&& body) {
-  Ctx m;
-  body(m);  // sizes
-  Ctx c;
-  c.run = true;
-  c.launch = true;
-  c.st = reinterpret_cast<hipStream_t>(stream);
-  uintptr_t b = reinterpret_cast<uintptr_t>(ws);
-  uintptr_t aligned = (b + 255) & ~uintptr_t(255);
-  const size_t lost = aligned - b;
-  ACETN_REQUIRE(ws != nullptr || (m.off == 0 && m.gwork_need == 0), "workspace is null");
-  ACETN_REQUIRE(ws_bytes >= lost, "workspace too small");
-  c.base = reinterpret_cast<char*>(aligned);
-  c.cap = ws_bytes - lost;
-  c.gwork_elems = m.gwork_need;
-  if (m.gwork_need) c.gwork = c.take(m.gwork_need);
-  body(c);
-}
-
-// Products issued inside a SplitScope run on the split-bf16 matrix cores at
-// the given level when the context is in mixed precision, else in FP64.
-struct SplitScope {
-  Ctx& c;
-  int saved;
-  SplitScope(Ctx& ctx, int level) : c(ctx), saved(ctx.split) { c.split = c.mixed ? level : 0; }
-  ~SplitScope() { c.split = saved; }
-};
-
-void gemm(Ctx& c, const GemmDesc& g, const double* A, const double* B, double* C) {
-  if (!c.run) {
-    c.gwork_need = std::max({c.gwork_need, gemm_f64_work_elems(g, 1), gemm_split_work_elems(g, 1)});
-    return;
-  }
-  if (!c.launch) return;
-  GemmBatch b{};
-  b.count = 1;
-  b.A[0] = A; b.B[0] = B; b.C[0] = C;
-  if (c.split > 0)
-    gemm_split(g, b, c.split, DType::F64, DType::F64, DType::F64, c.gwork, c.gwork_elems, c.st);
-  else
-    gemm_f64(g, b, c.gwork, c.gwork_elems, c.st);
-}
-
-// dense row-major helpers
-void mm(Ctx& c, int M, int N, int K, bool ta, const double* A, int64_t lda, bool tb,
-        const double* B, int64_t ldb, double* C, int64_t ldc, double alpha = 1.0,
-        double beta = 0.0) {
-  gemm(c, dense_desc(M, N, K, ta, lda, tb, ldb, ldc, alpha, beta), A, B, C);
-}
-
-void check_site(const acetn_site* s) {
-  ACETN_REQUIRE(s != nullptr && s->A != nullptr, "site is null");
-  ACETN_REQUIRE(s->D >= 1 && s->D <= 8 && s->d >= 1 && s->d <= 8, "unsupported D/d");
-  for (int k = 0; k < 4; ++k) {
-    ACETN_REQUIRE(s->C[k] && s->E[k], "site tensor pointer is null");
-    ACETN_REQUIRE(s->C_dim[k][0] > 0 && s->C_dim[k][1] > 0 && s->E_dim[k][0] > 0 &&
-                      s->E_dim[k][1] > 0, "non-positive boundary dimension");
-  }
-}
-
-// strides of bond_permute(A, k) axis j (A contiguous (D,D,D,D,d))
-void permuted_strides(const acetn_site* s, int k, int64_t out[4]) {
-  const int64_t D = s->D, d = s->d;
-  const int64_t as[4] = {D * D * D * d, D * D * d, D * d, d};
-  for (int j = 0; j < 4; ++j) out[j] = as[(j + k) % 4];
-}
 
 // ---------------------------------------------------------------------------
 // make_quarter_tensor (acetn/renormalization/projectors.py:35-60)
@@ -433,14 +330,6 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
 // ---------------------------------------------------------------------------
 // Reduced density matrices (acetn/measurement/rdm.py)
 // ---------------------------------------------------------------------------
-GemmDesc gdesc(int M, int N, int K, IdxMap am, IdxMap ak, IdxMap bk, IdxMap bn, IdxMap cm,
-               IdxMap cn) {
-  GemmDesc g{};
-  g.M = M; g.N = N; g.K = K;
-  g.am = am; g.ak = ak; g.bk = bk; g.bn = bn; g.cm = cm; g.cn = cn;
-  g.alpha = 1.0; g.beta = 0.0;
-  return g;
-}
 
 // build_site_rdm, rdm.py:35-67.  rho (d x d) = "Pp".
 void site_rdm(Ctx& c, const acetn_site* s, double* rho) {
@@ -511,14 +400,21 @@ void site_rdm(Ctx& c, const acetn_site* s, double* rho) {
 
 // build_bond_rdm + build_bond_rdm_core, rdm.py:69-120 (the blocked form of
 // :123-154 computes the same tensor).  rho (d,d,d,d) = "PQpq".
-void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double* rho) {
+}  // namespace
+
+// The two halves of the bond network (rdm.py:102-120; the same network
+// builds the full-update norm tensor, full_update.py:209-227):
+//   r1[f][c][L][P][l][p]  (site s1 with its C12, E12, E11, C13, E13)
+//   r2[f][c][R][Q][r][q]  (site s2 with its C21, E21, E24, C24, E23)
+// A1/A2 are addressed through their (l, u, r, d) leg strides ps1/ps2 in the
+// bond frame and a unit physical stride, so a reduced tensor a1q (r,d,u,y)
+// with y = (l, p) enters exactly like a site tensor.
+void bond_halves(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, const double* A1,
+                 const int64_t ps1[4], const double* A2, const int64_t ps2[4], BondHalves& out) {
   k &= 3;
   const int64_t D = s1->D, d = s1->d, D2 = D * D, d2 = d * d;
-  ACETN_REQUIRE(s2->D == D && s2->d == d, "bond rdm: site dimensions differ");
+  ACETN_REQUIRE(s2->D == D && s2->d == d, "bond network: site dimensions differ");
   const int k1 = (k + 1) & 3, k2 = (k + 2) & 3, k3 = (k + 3) & 3;
-  int64_t ps1[4], ps2[4];
-  permuted_strides(s1, k, ps1);
-  permuted_strides(s2, k, ps2);
   // ---------------- right half (site s1)
   const int64_t c12a = s1->C_dim[k1][0], c12b = s1->C_dim[k1][1];
   ACETN_REQUIRE(s1->E_dim[k1][0] == c12b, "bond rdm: c12/e12 mismatch");
@@ -544,7 +440,7 @@ void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double*
     // t[a=c][d][D][c=e][L][P][l][p]
     dl.y_o1 = D2 * Lr; dl.y_k4 = D * Lr; dl.y_b4 = Lr;
     dl.y_o2 = D2 * d2; dl.y_b3 = D * d2; dl.y_pb = D * d; dl.y_k3 = d; dl.y_p = 1;
-    double_layer_f64(dl, rt2, s1->A, t, c.st);
+    double_layer_f64(dl, rt2, A1, t, c.st);
   }
   const int64_t c13a = s1->C_dim[k2][0], c13b = s1->C_dim[k2][1];
   ACETN_REQUIRE(s1->E_dim[k2][0] == c13b, "bond rdm: c13/e13 mismatch");
@@ -581,7 +477,7 @@ void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double*
     // t2[e][d][D][c][R][Q][r][q]
     dl.y_o2 = D2 * Ll; dl.y_k4 = D * Ll; dl.y_b4 = Ll;
     dl.y_o1 = D2 * d2; dl.y_b3 = D * d2; dl.y_pb = D * d; dl.y_k3 = d; dl.y_p = 1;
-    double_layer_f64(dl, lt2, s2->A, t2, c.st);
+    double_layer_f64(dl, lt2, A2, t2, c.st);
   }
   const int64_t c24a = s2->C_dim[k3][0], c24b = s2->C_dim[k3][1];
   ACETN_REQUIRE(c24b == me, "bond rdm: c24/e24 mismatch");
@@ -593,34 +489,30 @@ void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double*
   double* r2 = c.take((size_t)F2 * Ll);
   gemm(c, gdesc((int)F2, (int)Ll, (int)(me * D2), plain(D2), grouped((int)D2, 1, F2 * D2),
                 plain(Ll), plain(1), plain(Ll), plain(1)), r2a, t2, r2);
+  ACETN_REQUIRE(F == F2 && ne == mc, "bond network: halves do not close");
+  out.r1 = r1;
+  out.r2 = r2;
+  out.F = F;
+  out.nc = ne;
+}
+
+namespace {
+
+void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double* rho) {
+  int64_t ps1[4], ps2[4];
+  permuted_strides(s1, k, ps1);
+  permuted_strides(s2, k, ps2);
+  BondHalves h{};
+  bond_halves(c, s1, s2, k, s1->A, ps1, s2->A, ps2, h);
+  const int64_t D = s1->D, d = s1->d, D2 = D * D, d2 = d * d;
+  const double* r1 = h.r1;
+  const double* r2 = h.r2;
   // ---------------- rho[P][Q][p][q] = sum_{f,c,R,r} r1[f][c][R][P][r][p] r2[f][c][R][Q][r][q]
-  ACETN_REQUIRE(F == F2 && ne == mc, "bond rdm: halves do not close");
-  const int64_t K = F * ne * D2;
+  const int64_t K = h.F * h.nc * D2;
   gemm(c, gdesc((int)d2, (int)d2, (int)K, grouped((int)d, 1, D * d),
                 grouped3((int)D, d, (int)D, D * d2, D2 * d2), grouped3((int)D, d, (int)D, D * d2, D2 * d2),
                 grouped((int)d, 1, D * d), grouped((int)d, d, d2 * d), grouped((int)d, 1, d2)),
        r1, r2, rho);
-}
-
-void check_precision(int p) {
-  ACETN_REQUIRE(p == ACETN_HIP_FP64 || p == ACETN_HIP_MIXED, "unknown precision mode");
-}
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const Error& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return 3;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return 3;
-  }
 }
 
 }  // namespace

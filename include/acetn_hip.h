@@ -149,6 +149,89 @@ size_t acetn_hip_bond_rdm_workspace(const acetn_site* s1, const acetn_site* s2, 
 int acetn_hip_bond_rdm(const acetn_site* s1, const acetn_site* s2, int k, double* rho,
                        void* workspace, size_t ws_bytes, void* stream);
 
+/* ---- full update (one bond) ---------------------------------------------
+ * Reduced bond dimension nD = d*D (min(D^3, dD) with D^2 >= d).  Bond-frame
+ * conventions of acetn/evolution/tensor_update.py:18-75: the bond (s1, s2, k)
+ * is rotated so that s2 sits on leg 2 ("right") of s1. */
+
+/* decompose_site_tensors, tensor_update.py:52-68: A1, A2 are the site
+ * tensors (D,D,D,D,d) in the site frame; outputs a1q (D,D,D,nD) (r,d,u,y),
+ * a1r (nD,D,d), a2q (D,D,D,nD) (d,l,u,x), a2r (nD,D,d).  Householder QR with
+ * LAPACK's sign conventions (those of the reference's torch.linalg.qr). */
+size_t acetn_hip_bond_decompose_workspace(int64_t D, int64_t d);
+int acetn_hip_bond_decompose(const double* A1, const double* A2, int k, int64_t D, int64_t d,
+                             double* a1q, double* a1r, double* a2q, double* a2r, void* workspace,
+                             size_t ws_bytes, void* stream);
+
+/* build_norm_tensor / build_norm_tensor_core, full_update.py:163-227:
+ * n12 (nD,nD,nD,nD) as [y][x][Y][X] from the environments of s1 and s2. */
+size_t acetn_hip_norm_tensor_workspace(const acetn_site* s1, const acetn_site* s2, int k);
+int acetn_hip_norm_tensor(const acetn_site* s1, const acetn_site* s2, int k, const double* a1q,
+                          const double* a2q, double* n12, void* workspace, size_t ws_bytes,
+                          void* stream);
+
+/* Gate application of FullUpdater.update_reduced_tensors,
+ * full_update.py:77-86: a12g[y][x][r][s] = sum a1r[y,u,p] a2r[x,u,q] gate[p,q,r,s]. */
+size_t acetn_hip_apply_gate_workspace(int64_t D, int64_t d);
+int acetn_hip_apply_gate(const double* a1r, const double* a2r, const double* gate, int64_t D,
+                         int64_t d, double* a12g, void* workspace, size_t ws_bytes, void* stream);
+
+/* positive_approx, full_update.py:262-297: nz (nD, nD, nD^2).  Synchronises
+ * the stream once per eigen-decomposition (the reference's loop condition);
+ * host_shifts receives the number of diagonal shifts applied. */
+size_t acetn_hip_positive_approx_workspace(int64_t D, int64_t d);
+int acetn_hip_positive_approx(const double* n12, int64_t D, int64_t d, double cutoff, double* nz,
+                              int32_t* host_shifts, void* workspace, size_t ws_bytes,
+                              void* stream);
+
+/* gauge_fix, full_update.py:300-343: from nz and a12g, the gauge-fixed n12
+ * and a12g and the pseudo-inverses (absolute cutoff atol) of the R factors. */
+size_t acetn_hip_gauge_fix_workspace(int64_t D, int64_t d);
+int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t D, int64_t d,
+                        double atol, double* n12, double* a12g, double* nzxr_inv,
+                        double* nzyr_inv, void* workspace, size_t ws_bytes, void* stream);
+
+/* ALS bond solve.  _initialize is the plain start of
+ * ALSSolver.initialize_tensors / initialize_reduced_tensors, als_solver.py:112-146.
+ * _solve replaces the extension's als_solve (csrc/evolution/als_solve.cpp:107-116,
+ * bound at acetn/evolution/als_solver.py:84-110) and solve_torch
+ * (als_solver.py:55-82): a1r/a2r carry the start in and the result out; the
+ * sweep loop and its stopping rule run on the device (one workgroup);
+ * host_sweeps/host_cost/host_info receive the sweep count, the final cost
+ * and flags (1: a Cholesky factorisation failed and LU with partial pivoting
+ * was used, as the reference's fallback; 2: a singular system). */
+enum { ACETN_HIP_ALS_CHOLESKY = 0, ACETN_HIP_ALS_PINV = 1 };
+size_t acetn_hip_als_workspace(int64_t D, int64_t d);
+int acetn_hip_als_initialize(const double* a12g, int64_t D, int64_t d, double* a1r, double* a2r,
+                             void* workspace, size_t ws_bytes, void* stream);
+int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, double* a2r,
+                        int64_t D, int64_t d, int niter, double tol, int method, double epsilon,
+                        int64_t* host_sweeps, double* host_cost, int32_t* host_info,
+                        void* workspace, size_t ws_bytes, void* stream);
+
+/* finalize_reduced_tensors (full_update.py:123-161; nzyr_inv/nzxr_inv null
+ * when the gauge fix is off) + recompose_site_tensors (tensor_update.py:70-75),
+ * each tensor divided by its norm and rotated back to the site frame
+ * (tensor_update.py:26-38): A1_out, A2_out (D,D,D,D,d). */
+size_t acetn_hip_finalize_bond_workspace(int64_t D, int64_t d);
+int acetn_hip_finalize_bond(const double* a1r, const double* a2r, const double* nzyr_inv,
+                            const double* nzxr_inv, const double* a1q, const double* a2q, int k,
+                            int64_t D, int64_t d, double* A1_out, double* A2_out, void* workspace,
+                            size_t ws_bytes, void* stream);
+
+/* SimpleUpdater.bond_update, acetn/evolution/simple_update.py:28-76, with
+ * LambdaTensors.absorb/strip (lambda_tensors.py:34-49): lam1[leg]/lam2[leg]
+ * are the D-vectors on the four legs of s1/s2 (null for the bond's own leg,
+ * i.e. all ones); A1_out/A2_out (site frame) and lam_out (D) receive the
+ * update.  The bond index's sign is fixed by the rule of
+ * acetn_hip_finalize_bond (a gauge choice; the reference leaves it to LAPACK). */
+size_t acetn_hip_simple_update_workspace(int64_t D, int64_t d);
+int acetn_hip_simple_update_bond(const double* A1, const double* A2, int k,
+                                 const double* const lam1[4], const double* const lam2[4],
+                                 const double* gate, int64_t D, int64_t d, double cutoff,
+                                 double* A1_out, double* A2_out, double* lam_out, void* workspace,
+                                 size_t ws_bytes, void* stream);
+
 /* ---- building blocks, exported for the parity tests ---------------------- */
 
 /* Gaussian sketch, n x q row-major (the device side of the generator the

@@ -337,6 +337,7 @@ def simple_update_bond(st, bond, gate, lam):
     Vh = Vh[:bD, :].reshape(bD, nD, pD)
     a1r = ein("ypa,a->yap", U, lt)
     a2r = ein("axq,a->xaq", Vh, lt)
+    a1r, a2r = fix_bond_gauge(a1r, a2r)   # bond-index sign gauge shared with the device
     a1, a2 = recompose_site_tensors(a1q, a1r, a2q, a2r)
     a1 = np.transpose(a1, bond_permutation(4 - k))
     a2 = np.transpose(a2, bond_permutation(4 - k))

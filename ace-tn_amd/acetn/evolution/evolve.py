@@ -22,8 +22,8 @@ def evolve(ipeps, dtau, steps, model, config, proposer=None):
     every sweep.  Returns the tensor updater (its statistics stay readable)."""
     gate = Gate(model, dtau, ipeps.bond_list, ipeps.site_list, device=ipeps.device)
     if proposer is None and getattr(config, "als_start", "plain") == "learned":
-        from .learned_start import default_proposer
-        proposer = default_proposer(ipeps)
+        raise NotImplementedError("evolution.als_start='learned' needs a trained proposer: "
+                                  "pass one to evolve(..., proposer=...)")
     tensor_updater = create_tensor_updater(ipeps, gate, config, proposer)
     loop_start = time.time()
     for _ in tqdm(range(1, steps + 1), desc="Updating tensors",

@@ -176,7 +176,22 @@ def main():
     ap.add_argument("--chi", type=int, default=144)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="run one precision only (profiling runs); prints no JSON line")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB once with a plain device copy (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calibration} MiB")
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -186,6 +201,9 @@ def main():
     _hip.require_gpu()
     log("device:", _hip.device_info())
     D, chi = args.D, args.chi
+    if args.only:
+        run_ctmrg(args.only, args)
+        return
     ref = run_ctmrg("fp64", args)
     mix = run_ctmrg("mixed", args)
     dE = abs(mix["energy"] - ref["energy"])

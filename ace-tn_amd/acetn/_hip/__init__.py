@@ -199,8 +199,20 @@ def slot_workspace(slot, nbytes, device):
     return w.get(nbytes, device)
 
 
+_serial = False
+
+
+def serialize(flag):
+    """When set, every slot runs on the caller's current stream (no overlap of
+    independent projector pairs); used to time kernels without contention."""
+    global _serial
+    _serial = bool(flag)
+
+
 def slot_stream(slot, device):
     """Default-priority side streams, one per concurrent slot."""
+    if _serial:
+        return torch.cuda.current_stream(device)
     key = (slot, str(device))
     if key not in _streams:
         _streams[key] = torch.cuda.Stream(device=device)

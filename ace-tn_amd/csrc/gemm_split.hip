@@ -1,13 +1,20 @@
 // Split-bf16 GEMM on the CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16) for
 // the mixed-precision CTMRG path.
 //
-// Each operand element x (FP32 or FP64 in HBM) is rounded to FP32 and split
-// into SPLIT bf16 pieces x = x0 + x1 (+ x2) with x_{i+1} = bf16(x - sum_{j<=i} x_j).
+// Each operand element x (FP64 in HBM) is rounded to FP32 and split into
+// SPLIT bf16 pieces x = x0 + x1 (+ x2) with x_{i+1} = bf16(x - sum_{j<=i} x_j).
 // The product keeps every piece pair (i, j) with i + j < SPLIT:
 //   SPLIT = 2 ("bf16x3"): 3 products, ~16-bit operands (rel. error ~1e-5)
 //   SPLIT = 3 ("bf16x6"): 6 products, ~24-bit operands (FP32-equivalent)
 // accumulated in FP32 by the MFMA.  bf16 products are exact in FP32, so the
 // only errors are the operand truncation and the FP32 accumulation.
+//
+// Operand B is split once per call, before the product, into SPLIT bf16
+// planes laid out [plane][n][k] with k contiguous (k padded to a multiple of
+// 32 with zeros): the product kernel then moves B with 16-byte loads straight
+// into LDS, and no M tile converts B again.  Operand A (the large enlarged-
+// corner matrix, read once per product) is converted and split in the
+// product kernel while it is staged through LDS.
 //
 // Tiles: 4 waves in a 2 x 2 grid, each owning WM x WN MFMA blocks of 16 x 16,
 // BK = 32 (one MFMA k-step).  LDS holds the pieces k-contiguous ([m][k] and
@@ -22,14 +29,12 @@ namespace acetn {
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 namespace {
 constexpr int BK = 32, NT = 256, KPAD = 8;
 
 __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; }
-
-template <typename T>
-__device__ __forceinline__ float ld(const T* p, int64_t off) { return (float)p[off]; }
 
 template <int SPLIT>
 __device__ __forceinline__ void split(float x, __bf16 (&o)[SPLIT]) {
@@ -41,26 +46,45 @@ __device__ __forceinline__ void split(float x, __bf16 (&o)[SPLIT]) {
   }
 }
 
-template <int WM, int WN, int SPLIT, typename TA, typename TB, typename TC>
-__global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, GemmBatch bt, int ksplit,
+// planes[s][n][k] (k padded to Kp) of B(k, n) = B[bk(k) + bn(n)]
+template <int SPLIT>
+__global__ void split_planes_kernel(const double* __restrict__ B, IdxMap bk, IdxMap bn, int K,
+                                    int N, int Kp, __bf16* __restrict__ planes) {
+  const int64_t total = (int64_t)N * Kp;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)(e / Kp), k = (int)(e % Kp);
+    const float x = (k < K) ? (float)B[imap(bk, k) + imap(bn, n)] : 0.f;
+    __bf16 pc[SPLIT];
+    split<SPLIT>(x, pc);
+#pragma unroll
+    for (int s = 0; s < SPLIT; ++s) planes[(int64_t)s * total + e] = pc[s];
+  }
+}
+
+template <int WM, int WN, int SPLIT>
+__global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double* __restrict__ A,
+                                                        const __bf16* __restrict__ Bp, int Kp,
+                                                        double* __restrict__ C, int ksplit,
                                                         double* __restrict__ work) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
-  constexpr int AE = BM * BK / NT, BE = BN * BK / NT;
+  constexpr int AE = BM * BK / NT;
   constexpr int LK = BK + KPAD;
+  constexpr int BOCT = SPLIT * BN * (BK / 8);          // 16-byte octets of B per k-tile
+  constexpr int BV = (BOCT + NT - 1) / NT;
   extern __shared__ __align__(16) char smem[];
   __bf16* As = reinterpret_cast<__bf16*>(smem);       // [SPLIT][BM][LK]
   __bf16* Bs = As + SPLIT * BM * LK;                   // [SPLIT][BN][LK]
   if (g.run_if && *g.run_if == 0) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int prob = blockIdx.z / ksplit, kpart = blockIdx.z % ksplit;
-  const TA* __restrict__ A = reinterpret_cast<const TA*>(bt.A[prob]);
-  const TB* __restrict__ B = reinterpret_cast<const TB*>(bt.B[prob]);
+  const int kpart = blockIdx.z;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   int kchunk = (g.K + ksplit - 1) / ksplit;
   kchunk = (kchunk + BK - 1) / BK * BK;
   const int kbeg = kpart * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
+  const int64_t plane = (int64_t)g.N * Kp;
 
   const bool a_mfast = contiguous(g.am) || !contiguous(g.ak);
   int a_ml[AE], a_kl[AE];
@@ -75,18 +99,16 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, GemmBatch bt
     a_mok[i] = m < g.M;
     a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
   }
-  const bool b_nfast = contiguous(g.bn) || !contiguous(g.bk);
-  int b_nl[BE], b_kl[BE];
-  int64_t b_noff[BE];
-  bool b_nok[BE];
+  // B octets: o -> (piece s, row nl, k-octet q)
+  int b_s[BV], b_nl[BV], b_q[BV];
+  bool b_ok[BV];
 #pragma unroll
-  for (int i = 0; i < BE; ++i) {
-    const int e = tid + i * NT;
-    if (b_nfast) { b_nl[i] = e % BN; b_kl[i] = e / BN; }
-    else         { b_kl[i] = e % BK; b_nl[i] = e / BK; }
-    const int n = n0 + b_nl[i];
-    b_nok[i] = n < g.N;
-    b_noff[i] = b_nok[i] ? imap(g.bn, n) : 0;
+  for (int i = 0; i < BV; ++i) {
+    const int o = tid + i * NT;
+    b_q[i] = o % (BK / 8);
+    b_nl[i] = (o / (BK / 8)) % BN;
+    b_s[i] = o / (BK / 8 * BN);
+    b_ok[i] = o < BOCT && n0 + b_nl[i] < g.N;
   }
 
   f4 acc[WM][WN];
@@ -95,17 +117,21 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, GemmBatch bt
 #pragma unroll
     for (int j = 0; j < WN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  float ra[AE], rb[BE];
+  float ra[AE];
+  u4 rb[BV];
   auto load_tile = [&](int kt) {
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
       const int k = kt + a_kl[i];
-      ra[i] = (a_mok[i] && k < kend) ? ld(A, a_moff[i] + imap(g.ak, k)) : 0.f;
+      ra[i] = (a_mok[i] && k < kend) ? (float)A[a_moff[i] + imap(g.ak, k)] : 0.f;
     }
 #pragma unroll
-    for (int i = 0; i < BE; ++i) {
-      const int k = kt + b_kl[i];
-      rb[i] = (b_nok[i] && k < kend) ? ld(B, imap(g.bk, k) + b_noff[i]) : 0.f;
+    for (int i = 0; i < BV; ++i) {
+      // the planes are zero beyond K (kt + 32 <= Kp always); k >= kend inside
+      // K belongs to another split-K part and is masked when stored to LDS
+      rb[i] = b_ok[i] ? *reinterpret_cast<const u4*>(
+                            &Bp[b_s[i] * plane + (int64_t)(n0 + b_nl[i]) * Kp + kt + 8 * b_q[i]])
+                      : u4{0u, 0u, 0u, 0u};
     }
   };
 
@@ -120,11 +146,20 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, GemmBatch bt
       for (int s = 0; s < SPLIT; ++s) As[(s * BM + a_ml[i]) * LK + a_kl[i]] = pc[s];
     }
 #pragma unroll
-    for (int i = 0; i < BE; ++i) {
-      __bf16 pc[SPLIT];
-      split<SPLIT>(rb[i], pc);
+    for (int i = 0; i < BV; ++i) {
+      if (tid + i * NT < BOCT) {
+        u4 v = rb[i];
+        const int k0 = kt + 8 * b_q[i];
+        if (k0 + 8 > kend) {
 #pragma unroll
-      for (int s = 0; s < SPLIT; ++s) Bs[(s * BN + b_nl[i]) * LK + b_kl[i]] = pc[s];
+          for (int t = 0; t < 4; ++t) {
+            const int ka = k0 + 2 * t;
+            if (ka >= kend) v[t] = 0u;
+            else if (ka + 1 >= kend) v[t] &= 0x0000FFFFu;
+          }
+        }
+        *reinterpret_cast<u4*>(&Bs[(b_s[i] * BN + b_nl[i]) * LK + 8 * b_q[i]]) = v;
+      }
     }
     __syncthreads();
     if (kt + BK < kend) load_tile(kt + BK);
@@ -166,29 +201,25 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, GemmBatch bt
         if (m >= g.M || n >= g.N) continue;
         const double v = (double)acc[i][j][r];
         if (ksplit == 1) {
-          TC* C = reinterpret_cast<TC*>(bt.C[prob]);
           const int64_t off = imap(g.cm, m) + imap(g.cn, n);
-          C[off] = (TC)((g.beta != 0.0) ? g.alpha * v + g.beta * (double)C[off] : g.alpha * v);
+          C[off] = (g.beta != 0.0) ? g.alpha * v + g.beta * C[off] : g.alpha * v;
         } else {
-          work[(((int64_t)prob * ksplit + kpart) * g.M + m) * g.N + n] = v;
+          work[((int64_t)kpart * g.M + m) * g.N + n] = v;
         }
       }
 }
 
-template <typename TC>
-__global__ void splitk_reduce_typed(GemmDesc g, GemmBatch bt, int ksplit,
+__global__ void splitk_reduce_split(GemmDesc g, double* __restrict__ C, int ksplit,
                                     const double* __restrict__ work) {
   if (g.run_if && *g.run_if == 0) return;
   const int64_t mn = (int64_t)g.M * g.N;
-  const int prob = blockIdx.y;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
        e += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int p = 0; p < ksplit; ++p) s += work[((int64_t)prob * ksplit + p) * mn + e];
+    for (int p = 0; p < ksplit; ++p) s += work[(int64_t)p * mn + e];
     const int m = (int)(e / g.N), n = (int)(e % g.N);
-    TC* C = reinterpret_cast<TC*>(bt.C[prob]);
     const int64_t off = imap(g.cm, m) + imap(g.cn, n);
-    C[off] = (TC)((g.beta != 0.0) ? g.alpha * s + g.beta * (double)C[off] : g.alpha * s);
+    C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
   }
 }
 
@@ -201,10 +232,10 @@ Tile choose_tile(const GemmDesc& g) {
   return Tile{2, 2};
 }
 
-int choose_ksplit(const GemmDesc& g, int count) {
+int choose_ksplit(const GemmDesc& g) {
   const Tile t = choose_tile(g);
   const int BM = 32 * t.wm, BN = 32 * t.wn;
-  const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * count;
+  const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
   if (tiles >= 512 || g.K < 8 * BK * 4) return 1;
   int ks = (int)((1024 + tiles - 1) / tiles);
   ks = std::min(ks, g.K / (BK * 4));
@@ -212,65 +243,83 @@ int choose_ksplit(const GemmDesc& g, int count) {
   return std::max(ks, 1);
 }
 
-template <int WM, int WN, int SPLIT, typename TA, typename TB, typename TC>
-void launch(const GemmDesc& g, const GemmBatch& b, int ks, double* work, hipStream_t s) {
+int padded_k(int K) { return (std::max(K, 1) + BK - 1) / BK * BK; }
+
+// scratch (in doubles): split-K partials, then the B planes
+// (rounded to 32 doubles so that the planes start 256-byte aligned)
+size_t partial_elems(const GemmDesc& g, int ks) {
+  return ks > 1 ? ((size_t)ks * g.M * g.N + 31) / 32 * 32 : 0;
+}
+size_t plane_elems(const GemmDesc& g, int split) {
+  const size_t bytes = (size_t)split * g.N * padded_k(g.K) * sizeof(__bf16);
+  return (bytes + 255) / 256 * 256 / sizeof(double);
+}
+
+template <int WM, int WN, int SPLIT>
+void launch(const GemmDesc& g, const double* A, const __bf16* Bp, int Kp, double* C, int ks,
+            double* work, hipStream_t s) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
   const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
   static bool attr = false;
   if (!attr) {
-    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)gemm_split_kernel<WM, WN, SPLIT, TA, TB, TC>,
+    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)gemm_split_kernel<WM, WN, SPLIT>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
-  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, b.count * ks);
+  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, ks);
   ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_split: grid too large");
   hipEvent_t e0 = kernel_timing_begin(s);
-  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT, TA, TB, TC>), grid, dim3(NT), lds, s, g, b,
+  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT>), grid, dim3(NT), lds, s, g, A, Bp, Kp, C,
                      ks, work);
   ACETN_HIP_CHECK(hipGetLastError());
-  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K * b.count);
+  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
   if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     const int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
-    hipLaunchKernelGGL(splitk_reduce_typed<TC>, dim3(blocks, b.count), dim3(256), 0, s, g, b, ks,
-                       work);
+    hipLaunchKernelGGL(splitk_reduce_split, dim3(blocks), dim3(256), 0, s, g, C, ks, work);
     ACETN_HIP_CHECK(hipGetLastError());
   }
 }
 
-template <int SPLIT, typename TA, typename TB, typename TC>
-void dispatch(const GemmDesc& g, const GemmBatch& b, int ks, double* work, hipStream_t s) {
+template <int SPLIT>
+void run(const GemmDesc& g, const double* A, const double* B, double* C, int ks, double* work,
+         hipStream_t s) {
+  const int Kp = padded_k(g.K);
+  __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, ks));
+  {
+    const int64_t total = (int64_t)g.N * Kp;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(split_planes_kernel<SPLIT>, dim3(blocks), dim3(256), 0, s, B, g.bk, g.bn,
+                       g.K, g.N, Kp, planes);
+    ACETN_HIP_CHECK(hipGetLastError());
+  }
   const Tile t = choose_tile(g);
-  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT, TA, TB, TC>(g, b, ks, work, s);
-  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT, TA, TB, TC>(g, b, ks, work, s);
-  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT, TA, TB, TC>(g, b, ks, work, s);
-  else launch<2, 2, SPLIT, TA, TB, TC>(g, b, ks, work, s);
+  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT>(g, A, planes, Kp, C, ks, work, s);
+  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT>(g, A, planes, Kp, C, ks, work, s);
+  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT>(g, A, planes, Kp, C, ks, work, s);
+  else launch<2, 2, SPLIT>(g, A, planes, Kp, C, ks, work, s);
 }
 }  // namespace
 
 size_t gemm_split_work_elems(const GemmDesc& g, int count) {
-  const int ks = choose_ksplit(g, count);
-  return ks > 1 ? (size_t)ks * count * g.M * g.N : 0;
+  ACETN_REQUIRE(count == 1, "gemm_split: batched products are not supported");
+  return partial_elems(g, choose_ksplit(g)) + plane_elems(g, 3);
 }
 
 void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DType tb, DType tc,
                 double* work, size_t work_elems, hipStream_t s) {
-  ACETN_REQUIRE(b.count >= 1 && b.count <= kMaxBatch, "gemm_split: bad batch count");
+  ACETN_REQUIRE(b.count == 1, "gemm_split: batched products are not supported");
   ACETN_REQUIRE(split == 2 || split == 3, "gemm_split: split must be 2 or 3");
+  ACETN_REQUIRE(ta == DType::F64 && tb == DType::F64 && tc == DType::F64,
+                "gemm_split: FP64 operands and result only");
   if (g.M <= 0 || g.N <= 0) return;
-  int ks = choose_ksplit(g, b.count);
-  if (ks > 1 && (work == nullptr || work_elems < (size_t)ks * b.count * g.M * g.N)) ks = 1;
-  const int key = (ta == DType::F32) * 4 + (tb == DType::F32) * 2 + (tc == DType::F32);
-#define ACETN_SPLIT_CASE(K, A_, B_, C_)                                          \
-  case K:                                                                        \
-    if (split == 2) dispatch<2, A_, B_, C_>(g, b, ks, work, s);                  \
-    else dispatch<3, A_, B_, C_>(g, b, ks, work, s);                             \
-    break;
-  switch (key) {
-    ACETN_SPLIT_CASE(0, double, double, double)
-    default: ACETN_REQUIRE(false, "gemm_split: unsupported operand types");
-  }
-#undef ACETN_SPLIT_CASE
+  ACETN_REQUIRE(g.K > 0 || g.beta == 0.0, "gemm_split: K == 0 with beta != 0 unsupported");
+  ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, 1) + plane_elems(g, split),
+                "gemm_split: scratch too small");
+  int ks = choose_ksplit(g);
+  if (work_elems < partial_elems(g, ks) + plane_elems(g, split)) ks = 1;
+  if (split == 2) run<2>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
+  else run<3>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
 }
 
 }  // namespace acetn

@@ -18,8 +18,10 @@ HIP bond reduced density matrices) are reported beside it.
 Prints ONE JSON line (rank 0).  Diagnostics go to stderr.
   roofline     -- the matrix-core kernel class with the most event-timed
                   time in the mixed run: summed algorithmic flops (2*M*N*K
-                  per GEMM launch; 4*d*D^6 per double-layer block) over the
-                  timed region / summed launch durations on their streams.
+                  per GEMM launch; 4*d*D^6 per double-layer block) / summed
+                  launch durations, from one extra iteration run after the
+                  timed region with the projector pairs on one stream and
+                  every launch bracketed by HIP events on that stream.
   cpu_baseline -- the FP64 numpy oracle (a restatement of the reference
                   torch path) on the host cores: one left move (2 of the 16
                   projector+renormalisation units of an iteration) on a
@@ -145,12 +147,21 @@ def run_ctmrg(precision, args):
         ctmrg_iteration(mover, net)
         log(f"[{precision}] warmup {i}: C0 {tuple(net[(0, 0)]['C'][0].shape)}")
     torch.cuda.synchronize()
-    _hip.kernel_timing(True)
+    _hip.serialize(args.serial)
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctmrg_iteration(mover, net)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # roofline pass: one more iteration with the projector pairs serialised on
+    # one stream, every matrix-core launch bracketed by HIP events on that
+    # stream (with overlapping streams an event interval would also count time
+    # spent waiting for compute units held by other streams' kernels)
+    _hip.serialize(True)
+    _hip.kernel_timing(True)
+    ctmrg_iteration(mover, net)
+    torch.cuda.synchronize()
+    _hip.serialize(args.serial)
     stats = {}
     for name, cls in (("gemm_f64_kernel", _hip.K_GEMM_F64),
                       ("gemm_split_kernel", _hip.K_GEMM_SPLIT),
@@ -176,6 +187,8 @@ def main():
     ap.add_argument("--chi", type=int, default=144)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the timed iterations with the projector pairs serialised")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="run one precision only (profiling runs); prints no JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

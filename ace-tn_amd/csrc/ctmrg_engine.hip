@@ -147,9 +147,23 @@ struct HalfOut {
   double* spectrum;
 };
 
+// A product with an enlarged corner as its A operand: from the corner's bf16
+// planes in mixed precision, from the FP64 matrix otherwise.
+void mm_corner(Ctx& c, const SplitPlanes& P, int M, int N, int K, bool ta, const double* A,
+               int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc) {
+  const GemmDesc g = dense_desc(M, N, K, ta, lda, false, ldb, ldc);
+  if (c.run && c.launch && c.split > 0 && P.p != nullptr) {
+    gemm_split_planar(g, P, B, C, c.split, c.gwork, c.gwork_elems, c.st);
+    return;
+  }
+  gemm(c, g, A, B, C);
+}
+
 // phase 0: everything with one host sync; phase 1: through the truncation
 // count (async copy of cD_new to pinned host memory); phase 2: the projector
 // products, given cD_new.  Phases 1 and 2 carve identical workspace layouts.
+// The workspace holds the bf16 planes of both corners in both orientations
+// whatever the precision, so one workspace query serves both.
 void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, int64_t chi,
                      int64_t oversampling, int niter, double cutoff, uint64_t seed,
                      uint64_t sstream, const HalfOut& out, int phase = 0) {
@@ -173,31 +187,49 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   int* cD_dev = c.take_int(2);
   double* X = c.take((size_t)q * q);
   double* Vp = c.take((size_t)n * q);
+  SplitPlanes q1n, q1t, q4n, q4t;
+  q1n.p = c.take(split_planes_elems((int)m, (int)kk));
+  q1t.p = c.take(split_planes_elems((int)kk, (int)m));
+  q4n.p = c.take(split_planes_elems((int)kk, (int)n));
+  q4t.p = c.take(split_planes_elems((int)n, (int)kk));
 
   const bool replay = c.run && phase == 2;
   if (replay) c.launch = false;
   quarter_tensor(c, s1, k, Q1);
   quarter_tensor(c, s4, k + 3, Q4);
+  if (c.mixed) {
+    if (c.launch) {
+      split_transpose(Q1, (int)m, (int)kk, kk, q1n, q1t, c.st);
+      split_transpose(Q4, (int)kk, (int)n, n, q4n, q4t, c.st);
+    } else if (c.run) {  // phase 2 replay: the planes are in the workspace already
+      q1n.rows = (int)m; q1n.kp = (int)((kk + 31) / 32 * 32); q1n.pieces = 3;
+      q1t.rows = (int)kk; q1t.kp = (int)((m + 31) / 32 * 32); q1t.pieces = 3;
+      q4n.rows = (int)kk; q4n.kp = (int)((n + 31) / 32 * 32); q4n.pieces = 3;
+      q4t.rows = (int)n; q4t.kp = (int)((kk + 31) / 32 * 32); q4t.pieces = 3;
+    }
+  } else {
+    q1n.p = q1t.p = q4n.p = q4t.p = nullptr;
+  }
   if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st);
   // mixed precision: the sketch and the first power iteration only steer the
   // range finder (bf16x3); later products carry the basis itself (bf16x6)
   SplitScope sc(c, 2);
   // Y = Q1 (Q4 Omega)
-  mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Om, q, W, q);
-  mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);
+  mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Om, q, W, q);
+  mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q);
   for (int it = 0; it < niter; ++it) {
     if (it >= 1) c.split = c.mixed ? 3 : 0;
     orthonormalize(c, Y, m, q);
-    mm(c, (int)kk, (int)q, (int)m, true, Q1, kk, false, Y, q, W, q);   // Q1^H Y
-    mm(c, (int)n, (int)q, (int)kk, true, Q4, n, false, W, q, Z, q);    // Q4^H (.)
-    mm(c, (int)kk, (int)q, (int)n, false, Q4, n, false, Z, q, W, q);   // Q4 (.)
-    mm(c, (int)m, (int)q, (int)kk, false, Q1, kk, false, W, q, Y, q);  // Q1 (.)
+    mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Y, q, W, q);   // Q1^H Y
+    mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, W, q, Z, q);    // Q4^H (.)
+    mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Z, q, W, q);   // Q4 (.)
+    mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q);  // Q1 (.)
   }
   c.split = c.mixed ? 3 : 0;
   orthonormalize(c, Y, m, q);
   // B^T = Q4^H (Q1^H Qm): n x q ; keep Wq = Q1^H Qm for P1
-  mm(c, (int)kk, (int)q, (int)m, true, Q1, kk, false, Y, q, Wq, q);
-  mm(c, (int)n, (int)q, (int)kk, true, Q4, n, false, Wq, q, BT, q);
+  mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Y, q, Wq, q);
+  mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, Wq, q, BT, q);
   small_svd(c, BT, n, q, w, UB);
   if (!c.run) {
     // size the split-K scratch of the post-truncation products at cD = chi
@@ -233,7 +265,7 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   scale_columns(UB, (int)q, s, cD, 1, X, c.st);
   mm(c, (int)n, cD, (int)q, false, BT, q, false, X, cD, Vp, cD);
   c.split = c.mixed ? 3 : 0;
-  mm(c, (int)kk, cD, (int)n, false, Q4, n, false, Vp, cD, out.P2, cD);
+  mm_corner(c, q4n, (int)kk, cD, (int)n, false, Q4, n, Vp, cD, out.P2, cD);
   fix_projector_gauge(out.P1, out.P2, kk, kk, cD, c.st);
 }
 

@@ -12,9 +12,11 @@
 // Operand B is split once per call, before the product, into SPLIT bf16
 // planes laid out [plane][n][k] with k contiguous (k padded to a multiple of
 // 32 with zeros): the product kernel then moves B with 16-byte loads straight
-// into LDS, and no M tile converts B again.  Operand A (the large enlarged-
-// corner matrix, read once per product) is converted and split in the
-// product kernel while it is staged through LDS.
+// into LDS, and no M tile converts B again.  Operand A is either converted
+// and split in the product kernel while it is staged through LDS, or given
+// already split as planes [plane][m][k] (split_transpose_planes produces both
+// orientations of a matrix in one pass, for a matrix that several products
+// read, such as an enlarged corner in the range finder).
 //
 // Tiles: 4 waves in a 2 x 2 grid, each owning WM x WN MFMA blocks of 16 x 16,
 // BK = 32 (one MFMA k-step).  LDS holds the pieces k-contiguous ([m][k] and
@@ -62,8 +64,43 @@ __global__ void split_planes_kernel(const double* __restrict__ B, IdxMap bk, Idx
   }
 }
 
-template <int WM, int WN, int SPLIT>
+// X (R x Cc, row-major, leading dimension ld) -> planes of both orientations:
+// PN[s][r][c] (Cp columns, zero padded) and PT[s][c][r] (Rp columns).  One
+// 64 x 64 tile per block, transposed through LDS.
+constexpr int TT = 64;
+__global__ __launch_bounds__(256) void split_transpose_kernel(const double* __restrict__ X, int R,
+                                                              int Cc, int64_t ld, int Rp, int Cp,
+                                                              __bf16* __restrict__ PN,
+                                                              __bf16* __restrict__ PT) {
+  __shared__ __bf16 tile[3][TT][TT + 8];
+  const int r0 = blockIdx.y * TT, c0 = blockIdx.x * TT;
+  const int64_t pn = (int64_t)R * Cp, pt = (int64_t)Cc * Rp;
+  for (int e = threadIdx.x; e < TT * TT; e += 256) {
+    const int rr = e / TT, cc = e % TT;
+    const int r = r0 + rr, c = c0 + cc;
+    const float x = (r < R && c < Cc) ? (float)X[(int64_t)r * ld + c] : 0.f;
+    __bf16 pc[3];
+    split<3>(x, pc);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      tile[s][rr][cc] = pc[s];
+      if (r < R && c < Cp) PN[s * pn + (int64_t)r * Cp + c] = pc[s];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TT * TT; e += 256) {
+    const int cc = e / TT, rr = e % TT;
+    const int r = r0 + rr, c = c0 + cc;
+    if (c < Cc && r < Rp) {
+#pragma unroll
+      for (int s = 0; s < 3; ++s) PT[s * pt + (int64_t)c * Rp + r] = tile[s][rr][cc];
+    }
+  }
+}
+
+template <int WM, int WN, int SPLIT, bool APL>
 __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double* __restrict__ A,
+                                                        const __bf16* __restrict__ Ap, int64_t aplane,
                                                         const __bf16* __restrict__ Bp, int Kp,
                                                         double* __restrict__ C, int ksplit,
                                                         double* __restrict__ work) {
@@ -72,6 +109,9 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
   constexpr int LK = BK + KPAD;
   constexpr int BOCT = SPLIT * BN * (BK / 8);          // 16-byte octets of B per k-tile
   constexpr int BV = (BOCT + NT - 1) / NT;
+  constexpr int AOCT = SPLIT * BM * (BK / 8);          // same for planar A
+  constexpr int AV = APL ? (AOCT + NT - 1) / NT : 1;
+  constexpr int AEe = APL ? 1 : AE;
   extern __shared__ __align__(16) char smem[];
   __bf16* As = reinterpret_cast<__bf16*>(smem);       // [SPLIT][BM][LK]
   __bf16* Bs = As + SPLIT * BM * LK;                   // [SPLIT][BN][LK]
@@ -87,17 +127,31 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
   const int64_t plane = (int64_t)g.N * Kp;
 
   const bool a_mfast = contiguous(g.am) || !contiguous(g.ak);
-  int a_ml[AE], a_kl[AE];
-  int64_t a_moff[AE];
-  bool a_mok[AE];
+  int a_ml[AEe], a_kl[AEe];
+  int64_t a_moff[AEe];
+  bool a_mok[AEe];
+  if (!APL) {
 #pragma unroll
-  for (int i = 0; i < AE; ++i) {
-    const int e = tid + i * NT;
-    if (a_mfast) { a_ml[i] = e % BM; a_kl[i] = e / BM; }
-    else         { a_kl[i] = e % BK; a_ml[i] = e / BK; }
-    const int m = m0 + a_ml[i];
-    a_mok[i] = m < g.M;
-    a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
+    for (int i = 0; i < AEe; ++i) {
+      const int e = tid + i * NT;
+      if (a_mfast) { a_ml[i] = e % BM; a_kl[i] = e / BM; }
+      else         { a_kl[i] = e % BK; a_ml[i] = e / BK; }
+      const int m = m0 + a_ml[i];
+      a_mok[i] = m < g.M;
+      a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
+    }
+  }
+  int pa_s[AV], pa_ml[AV], pa_q[AV];
+  bool pa_ok[AV];
+  if (APL) {
+#pragma unroll
+    for (int i = 0; i < AV; ++i) {
+      const int o = tid + i * NT;
+      pa_q[i] = o % (BK / 8);
+      pa_ml[i] = (o / (BK / 8)) % BM;
+      pa_s[i] = o / (BK / 8 * BM);
+      pa_ok[i] = o < AOCT && m0 + pa_ml[i] < g.M;
+    }
   }
   // B octets: o -> (piece s, row nl, k-octet q)
   int b_s[BV], b_nl[BV], b_q[BV];
@@ -117,13 +171,22 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
 #pragma unroll
     for (int j = 0; j < WN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  float ra[AE];
+  float ra[AEe];
+  u4 rpa[AV];
   u4 rb[BV];
   auto load_tile = [&](int kt) {
+    if (APL) {
 #pragma unroll
-    for (int i = 0; i < AE; ++i) {
-      const int k = kt + a_kl[i];
-      ra[i] = (a_mok[i] && k < kend) ? (float)A[a_moff[i] + imap(g.ak, k)] : 0.f;
+      for (int i = 0; i < AV; ++i)
+        rpa[i] = pa_ok[i] ? *reinterpret_cast<const u4*>(
+                                &Ap[pa_s[i] * aplane + (int64_t)(m0 + pa_ml[i]) * Kp + kt + 8 * pa_q[i]])
+                          : u4{0u, 0u, 0u, 0u};
+    } else {
+#pragma unroll
+      for (int i = 0; i < AEe; ++i) {
+        const int k = kt + a_kl[i];
+        ra[i] = (a_mok[i] && k < kend) ? (float)A[a_moff[i] + imap(g.ak, k)] : 0.f;
+      }
     }
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
@@ -138,12 +201,21 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
   if (kbeg < kend) load_tile(kbeg);
   for (int kt = kbeg; kt < kend; kt += BK) {
     __syncthreads();
+    if (APL) {
+      // A and B planes are zero beyond K; inside K, k >= kend is zeroed in B
+      // only (one zero factor per product suffices)
 #pragma unroll
-    for (int i = 0; i < AE; ++i) {
-      __bf16 pc[SPLIT];
-      split<SPLIT>(ra[i], pc);
+      for (int i = 0; i < AV; ++i)
+        if (tid + i * NT < AOCT)
+          *reinterpret_cast<u4*>(&As[(pa_s[i] * BM + pa_ml[i]) * LK + 8 * pa_q[i]]) = rpa[i];
+    } else {
 #pragma unroll
-      for (int s = 0; s < SPLIT; ++s) As[(s * BM + a_ml[i]) * LK + a_kl[i]] = pc[s];
+      for (int i = 0; i < AEe; ++i) {
+        __bf16 pc[SPLIT];
+        split<SPLIT>(ra[i], pc);
+#pragma unroll
+        for (int s = 0; s < SPLIT; ++s) As[(s * BM + a_ml[i]) * LK + a_kl[i]] = pc[s];
+      }
     }
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
@@ -255,22 +327,22 @@ size_t plane_elems(const GemmDesc& g, int split) {
   return (bytes + 255) / 256 * 256 / sizeof(double);
 }
 
-template <int WM, int WN, int SPLIT>
-void launch(const GemmDesc& g, const double* A, const __bf16* Bp, int Kp, double* C, int ks,
-            double* work, hipStream_t s) {
+template <int WM, int WN, int SPLIT, bool APL>
+void launch(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, const __bf16* Bp,
+            int Kp, double* C, int ks, double* work, hipStream_t s) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
   const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
   static bool attr = false;
   if (!attr) {
-    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)gemm_split_kernel<WM, WN, SPLIT>,
+    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)gemm_split_kernel<WM, WN, SPLIT, APL>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
   dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, ks);
   ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_split: grid too large");
   hipEvent_t e0 = kernel_timing_begin(s);
-  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT>), grid, dim3(NT), lds, s, g, A, Bp, Kp, C,
-                     ks, work);
+  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT, APL>), grid, dim3(NT), lds, s, g, A, Ap,
+                     aplane, Bp, Kp, C, ks, work);
   ACETN_HIP_CHECK(hipGetLastError());
   kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
   if (ks > 1) {
@@ -281,9 +353,9 @@ void launch(const GemmDesc& g, const double* A, const __bf16* Bp, int Kp, double
   }
 }
 
-template <int SPLIT>
-void run(const GemmDesc& g, const double* A, const double* B, double* C, int ks, double* work,
-         hipStream_t s) {
+template <int SPLIT, bool APL>
+void run(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, const double* B,
+         double* C, int ks, double* work, hipStream_t s) {
   const int Kp = padded_k(g.K);
   __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, ks));
   {
@@ -294,10 +366,10 @@ void run(const GemmDesc& g, const double* A, const double* B, double* C, int ks,
     ACETN_HIP_CHECK(hipGetLastError());
   }
   const Tile t = choose_tile(g);
-  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT>(g, A, planes, Kp, C, ks, work, s);
-  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT>(g, A, planes, Kp, C, ks, work, s);
-  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT>(g, A, planes, Kp, C, ks, work, s);
-  else launch<2, 2, SPLIT>(g, A, planes, Kp, C, ks, work, s);
+  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
+  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
+  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
+  else launch<2, 2, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
 }
 }  // namespace
 
@@ -318,8 +390,40 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
                 "gemm_split: scratch too small");
   int ks = choose_ksplit(g);
   if (work_elems < partial_elems(g, ks) + plane_elems(g, split)) ks = 1;
-  if (split == 2) run<2>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
-  else run<3>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
+  if (split == 2) run<2, false>(g, b.A[0], nullptr, 0, b.B[0], b.C[0], ks, work, s);
+  else run<3, false>(g, b.A[0], nullptr, 0, b.B[0], b.C[0], ks, work, s);
+}
+
+void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B, double* C,
+                       int split, double* work, size_t work_elems, hipStream_t s) {
+  ACETN_REQUIRE(split == 2 || split == 3, "gemm_split: split must be 2 or 3");
+  ACETN_REQUIRE(A.p != nullptr && A.rows == g.M && A.kp == padded_k(g.K) && A.pieces >= split,
+                "gemm_split: A planes do not match the product");
+  if (g.M <= 0 || g.N <= 0) return;
+  ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, 1) + plane_elems(g, split),
+                "gemm_split: scratch too small");
+  int ks = choose_ksplit(g);
+  if (work_elems < partial_elems(g, ks) + plane_elems(g, split)) ks = 1;
+  const int64_t aplane = (int64_t)A.rows * A.kp;
+  const __bf16* Ap = reinterpret_cast<const __bf16*>(A.p);
+  if (split == 2) run<2, true>(g, nullptr, Ap, aplane, B, C, ks, work, s);
+  else run<3, true>(g, nullptr, Ap, aplane, B, C, ks, work, s);
+}
+
+size_t split_planes_elems(int rows, int K) {
+  const size_t bytes = (size_t)3 * rows * padded_k(K) * sizeof(__bf16);
+  return (bytes + 255) / 256 * 256 / sizeof(double);
+}
+
+void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N, SplitPlanes& T,
+                     hipStream_t s) {
+  ACETN_REQUIRE(N.p && T.p, "split_transpose: null planes");
+  N.rows = R; N.kp = padded_k(Cc); N.pieces = 3;
+  T.rows = Cc; T.kp = padded_k(R); T.pieces = 3;
+  dim3 grid((std::max(N.kp, Cc) + TT - 1) / TT, (std::max(T.kp, R) + TT - 1) / TT);
+  hipLaunchKernelGGL(split_transpose_kernel, grid, dim3(256), 0, s, X, R, Cc, ld, T.kp, N.kp,
+                     reinterpret_cast<__bf16*>(N.p), reinterpret_cast<__bf16*>(T.p));
+  ACETN_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace acetn

@@ -105,6 +105,22 @@ enum class DType : int { F64 = 0, F32 = 1 };
 void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DType tb, DType tc,
                 double* work, size_t work_elems, hipStream_t s);
 size_t gemm_split_work_elems(const GemmDesc& g, int count);
+// A matrix already split into 3 bf16 planes [piece][rows][kp] (k contiguous,
+// kp = K rounded up to 32, zero padded).  p points at device memory of
+// split_planes_elems(rows, K) doubles.
+struct SplitPlanes {
+  void* p = nullptr;
+  int rows = 0, kp = 0, pieces = 0;
+};
+size_t split_planes_elems(int rows, int K);
+// Both orientations of X (R x Cc, row-major, leading dimension ld): N holds X
+// itself (rows R, k = columns), T holds X^T (rows Cc, k = rows of X).
+void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N, SplitPlanes& T,
+                     hipStream_t s);
+// C = alpha A B + beta C with A given as planes (g.am/g.ak unused; A(m, k) =
+// the planes' row m, column k).
+void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B, double* C,
+                       int split, double* work, size_t work_elems, hipStream_t s);
 
 // Live timing of the matrix-core kernels (HIP events on the launch stream),
 // per kernel class; the benchmark's roofline figures come from here.

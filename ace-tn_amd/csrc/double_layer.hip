@@ -23,7 +23,11 @@ namespace acetn {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace {
-constexpr int NT = 256;
+// 8 waves per workgroup: the LDS footprint (A, the stacked X tile and the
+// intermediate T) allows one workgroup per CU at D = 6, so the waves of that
+// one workgroup are what hides latency.
+constexpr int NT = 512;
+constexpr int NW = NT / 64;
 
 template <int D>
 struct DLShape {
@@ -34,7 +38,8 @@ struct DLShape {
   static constexpr int KP = (D2 + 3) / 4 * 4;      // padded contraction length
   static constexpr int NP = (D2 + 15) / 16 * 16;   // padded (b3,b4) columns / step d rows
   static constexpr int TC = (MP / 16) * (NP / 16); // tiles per product
-  static constexpr int TPW = (TC + 3) / 4;         // tiles per wave
+  static constexpr int TPW = (TC + NW - 1) / NW;   // tiles per wave
+  static constexpr int XPT = (MP * KP + NT - 1) / NT; // X elements per thread
 };
 
 template <int D>
@@ -62,22 +67,50 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
 
   const int nblk = g.n1 * g.n2;
   const int ngroups = (nblk + S::NB - 1) / S::NB;
+  // X tile of a group, element e = (row, col) of the stacked [MP][KP] tile;
+  // loaded into registers one group ahead of its use.  The in-block part of
+  // each element's offset is fixed, only the block base changes per group.
+  double xr[S::XPT];
+  int x_loff[S::XPT];
+  int x_blk[S::XPT];
+#pragma unroll
+  for (int t = 0; t < S::XPT; ++t) {
+    const int e = tid + t * NT;
+    const int row = e / S::KP, col = e % S::KP;
+    const int blk = row / S::D2, kk = row % S::D2;
+    const bool ok = e < S::MP * S::KP && row < S::ROWS && col < S::D2;
+    const int k1 = kk / D, k2 = kk % D, b1 = col / D, b2 = col % D;
+    x_loff[t] = ok ? (int)(k1 * g.x_k1 + b1 * g.x_b1 + k2 * g.x_k2 + b2 * g.x_b2) : 0;
+    x_blk[t] = ok ? blk : -1;
+  }
+  auto load_x = [&](int grp) {
+    const int b0 = grp * S::NB;
+    int64_t base[S::NB];
+#pragma unroll
+    for (int bb = 0; bb < S::NB; ++bb) {
+      const int b = b0 + bb;
+      base[bb] = (b < nblk) ? (int64_t)(b / g.n2) * g.x_o1 + (int64_t)(b % g.n2) * g.x_o2 : -1;
+    }
+#pragma unroll
+    for (int t = 0; t < S::XPT; ++t) {
+      int64_t bo = -1;
+#pragma unroll
+      for (int bb = 0; bb < S::NB; ++bb)
+        if (x_blk[t] == bb) bo = base[bb];
+      xr[t] = (bo >= 0) ? X[bo + x_loff[t]] : 0.0;
+    }
+  };
+  if ((int)blockIdx.x < ngroups) load_x(blockIdx.x);
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int b0 = grp * S::NB;
     __syncthreads();
-    // ---- stacked X tile
-    for (int e = tid; e < S::MP * S::KP; e += NT) {
-      const int row = e / S::KP, col = e % S::KP;
-      double v = 0.0;
-      const int blk = row / S::D2, kk = row % S::D2;
-      if (row < S::ROWS && col < S::D2 && b0 + blk < nblk) {
-        const int o1 = (b0 + blk) / g.n2, o2 = (b0 + blk) % g.n2;
-        const int k1 = kk / D, k2 = kk % D, b1 = col / D, b2 = col % D;
-        v = X[(int64_t)o1 * g.x_o1 + (int64_t)o2 * g.x_o2 + k1 * g.x_k1 + b1 * g.x_b1 +
-              k2 * g.x_k2 + b2 * g.x_b2];
-      }
-      Xs[e] = v;
+    // ---- stacked X tile (from registers), then start loading the next group
+#pragma unroll
+    for (int t = 0; t < S::XPT; ++t) {
+      const int e = tid + t * NT;
+      if (e < S::MP * S::KP) Xs[e] = xr[t];
     }
+    if (grp + (int)gridDim.x < ngroups) load_x(grp + gridDim.x);
     d4 yacc[S::TPW];
 #pragma unroll
     for (int t = 0; t < S::TPW; ++t) yacc[t] = d4{0.0, 0.0, 0.0, 0.0};
@@ -93,6 +126,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
         if (tile < S::TC) {
           const int tm = tile / (S::NP / 16), tn = tile % (S::NP / 16);
           d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 3
           for (int ks = 0; ks < S::KP; ks += 4) {
             const double a = Xs[(tm * 16 + (lane & 15)) * S::KP + ks + (lane >> 4)];
             const double b = Apb[(ks + (lane >> 4)) * S::NP + tn * 16 + (lane & 15)];
@@ -118,6 +152,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
             const int tm = tile / (S::MP / 16), tn = tile % (S::MP / 16);
             const int nn = tn * 16 + (lane & 15);
             const int blk = nn / S::D2, b34 = nn % S::D2;
+#pragma unroll 3
             for (int ks = 0; ks < S::KP; ks += 4) {
               const int kk = ks + (lane >> 4);
               const double a = Ap[kk * S::NP + tm * 16 + (lane & 15)];
@@ -164,6 +199,8 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
   const size_t lds = sizeof(double) * ((size_t)g.d * S::KP * S::NP + (size_t)S::MP * S::KP +
                                        (size_t)S::MP * S::NP);
   ACETN_REQUIRE(lds <= 160 * 1024, "double_layer: LDS budget exceeded");
+  ACETN_REQUIRE((int64_t)(D - 1) * (g.x_k1 + g.x_b1 + g.x_k2 + g.x_b2) < (int64_t(1) << 31),
+                "double_layer: block strides exceed 32-bit offsets");
   static bool attr = false;
   if (!attr) {
     ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)dl_mfma_kernel<D>,

@@ -60,6 +60,17 @@ def _declare(lib):
                                                          ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                          c_i64, c_i64, ctypes.c_int, ctypes.c_int,
                                                          c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_projectors_workspace": (c_size, [ctypes.POINTER(AcetnSite) * 4, ctypes.c_int,
+                                                    c_i64, c_i64, ctypes.c_int, ctypes.c_int]),
+        "acetn_hip_projectors_begin": (ctypes.c_int, [ctypes.POINTER(AcetnSite) * 4, ctypes.c_int,
+                                                      c_i64, c_i64, ctypes.c_int, c_dbl, c_u64,
+                                                      c_u64, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_int, c_vp, c_vp, c_vp, c_size,
+                                                      c_vp]),
+        "acetn_hip_projectors_end": (ctypes.c_int, [ctypes.POINTER(AcetnSite) * 4, ctypes.c_int,
+                                                    c_i64, c_i64, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_int, c_i64, c_vp, c_vp,
+                                                    c_vp, c_size, c_vp]),
         "acetn_hip_renormalize_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                      c_i64, c_i64]),
         "acetn_hip_renormalize_boundary": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
@@ -355,30 +366,39 @@ def half_projectors(site1, site4, k, chi, oversampling, niter, cutoff, seed, str
     return P1[:rows * n].view(rows, n), P2[:rows * n].view(rows, n), spec
 
 
-class ProjectorJob:
-    """One half-system projector pair computed in two phases (see
-    acetn_hip_half_projectors_begin/_end)."""
+SYSTEMS = {"half-system": 0, "full-system": 1}
+SVD_TYPES = {"rsvd": 0, "full-rank": 1}
 
-    def __init__(self, slot, site1, site4, k, chi, oversampling, niter, cutoff, seed, stream_id,
-                 precision="fp64"):
-        self.s1, self.keep1 = make_site(*site1)
-        self.s4, self.keep4 = make_site(*site4)
+
+class ProjectorJob:
+    """One projector pair computed in two phases (acetn_hip_projectors_begin
+    / _end): ``sites`` are the four plaquette sites (A, C, E) in s1..s4
+    order; system 'half-system' | 'full-system', svd 'rsvd' | 'full-rank'."""
+
+    def __init__(self, slot, sites, k, chi, oversampling, niter, cutoff, seed, stream_id,
+                 precision="fp64", system="half-system", svd="rsvd"):
+        made = [make_site(*st) for st in sites]
+        self.sv = [m[0] for m in made]
+        self.keep = [m[1] for m in made]
+        self.arr = (ctypes.POINTER(AcetnSite) * 4)(*[ctypes.pointer(x) for x in self.sv])
         self.k, self.chi, self.p, self.niter = k, chi, oversampling, niter
         self.prec = precision_code(precision)
-        self.dev = self.keep1[0].device
-        self.rows = self.s1.E_dim[(3 + k) % 4][0] * self.s1.D * self.s1.D
+        self.system, self.svd = SYSTEMS[system], SVD_TYPES[svd]
+        s1 = self.sv[0]
+        self.dev = self.keep[0][0].device
+        self.rows = s1.E_dim[(3 + k) % 4][0] * s1.D * s1.D
         self.slot = slot
         self.stream = slot_stream(slot, self.dev)
-        nb = lib().acetn_hip_half_projectors_workspace(ctypes.byref(self.s1), ctypes.byref(self.s4),
-                                                       k, chi, oversampling)
+        nb = lib().acetn_hip_projectors_workspace(self.arr, k, chi, oversampling, self.system,
+                                                  self.svd)
         self.ws = slot_workspace(slot, nb, self.dev)
         self.cd_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
-        self.spec = torch.zeros(chi + oversampling, dtype=torch.float64, device=self.dev)
+        nspec = max(chi + oversampling, 512) if self.svd else chi + oversampling
+        self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
         with torch.cuda.stream(self.stream):
-            check(lib().acetn_hip_half_projectors_begin(
-                ctypes.byref(self.s1), ctypes.byref(self.s4), k, chi, oversampling, niter, cutoff,
-                seed, stream_id, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec),
-                ptr(self.ws),
+            check(lib().acetn_hip_projectors_begin(
+                self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
+                self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec), ptr(self.ws),
                 self.ws.numel(), c_vp(self.stream.cuda_stream)))
 
     def finish(self):
@@ -387,10 +407,9 @@ class ProjectorJob:
         with torch.cuda.stream(self.stream):
             P1 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)
             P2 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)
-            check(lib().acetn_hip_half_projectors_end(
-                ctypes.byref(self.s1), ctypes.byref(self.s4), self.k, self.chi, self.p, self.niter,
-                self.prec, n, ptr(P1), ptr(P2), ptr(self.ws), self.ws.numel(),
-                c_vp(self.stream.cuda_stream)))
+            check(lib().acetn_hip_projectors_end(
+                self.arr, self.k, self.chi, self.p, self.niter, self.system, self.svd, self.prec,
+                n, ptr(P1), ptr(P2), ptr(self.ws), self.ws.numel(), c_vp(self.stream.cuda_stream)))
         return P1, P2
 
 

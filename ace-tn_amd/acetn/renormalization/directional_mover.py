@@ -33,7 +33,16 @@ class DirectionalMover:
         x reads C0,C3,E0,E2,E3 of column x and writes C0,C3,E3 of column x+1,
         its right partner reads C1,C2,E0,E1,E2 and writes C1,C2,E1 -- so the
         result equals the sequential order.  Sketch streams are numbered in
-        that sequential order."""
+        that sequential order.  With full-system projectors the right move
+        reads corners the left move rewrites, so the moves run one after
+        the other (each still with its projector pairs concurrent)."""
+        if self.projector_calculator.projectors != "half-system" and len(moves) > 1:
+            for m in moves:
+                self._stage(ipeps, [m])
+            return
+        self._stage(ipeps, moves)
+
+    def _stage(self, ipeps, moves):
         import torch
         jobs, targets = [], []
         for line, k in moves:

@@ -48,46 +48,35 @@ class ProjectorCalculator:
         return st['A'], st['C'], st['E']
 
     def calculate_half_system(self, ipeps, sites, k):
-        """projectors.py:138-174 (rsvd branch)."""
-        sites = [tuple(s) for s in sites]
-        for s in sites:
-            ipeps[s]  # raises ValueError for an unknown site
-        if self.svd_type != "rsvd":
-            raise NotImplementedError("svd_type='full-rank' is not provided by the HIP engine; "
-                                      "use 'rsvd'")
-        s1, s4 = sites[0], sites[3]
-        chi = ipeps.dims['chi']
-        P1, P2, spec = _hip.half_projectors(self._site(ipeps, s1), self._site(ipeps, s4), k % 4,
-                                            chi, self.rsvd_oversampling, self.rsvd_niter,
-                                            self.svd_cutoff, self.sketch.seed, self.sketch.take(),
-                                            precision=self.precision)
-        self.last_spectrum = spec
-        D = ipeps.dims['bond']
-        rows = P1.shape[0]
-        return P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)
+        """projectors.py:138-174 (rsvd) / :62-82 + :114-136 (full-rank)."""
+        return self.calculate_many(ipeps, [(sites, k)], system="half-system")[0]
 
-    def calculate_many(self, ipeps, jobs):
+    def calculate_full_system(self, ipeps, sites, k):
+        """projectors.py:176-217 (rsvd) / :84-112 + :114-136 (full-rank)."""
+        return self.calculate_many(ipeps, [(sites, k)], system="full-system")[0]
+
+    def calculate_many(self, ipeps, jobs, system=None):
         """Projector pairs of mutually independent plaquettes, ``jobs`` a list
         of (sites, k) in the reference's call order (which fixes their sketch
         streams).  Each pair runs on its own stream; one host synchronisation
         reads all truncation ranks."""
-        if self.projectors != "half-system":
-            return [self.calculate(ipeps, sites, k) for sites, k in jobs]
-        if self.svd_type != "rsvd":
-            raise NotImplementedError("svd_type='full-rank' is not provided by the HIP engine; "
-                                      "use 'rsvd'")
         import torch
+        system = system or ("full-system" if self.calculate == self.calculate_full_system
+                            else "half-system")
+        if self.svd_type not in ("rsvd", "full-rank"):
+            raise ValueError(f"Invalid svd_type: {self.svd_type}")
         chi, D = ipeps.dims['chi'], ipeps.dims['bond']
         started = []
         for slot, (sites, k) in enumerate(jobs):
             sites = [tuple(s) for s in sites]
             for s in sites:
-                ipeps[s]
-            started.append(_hip.ProjectorJob(slot, self._site(ipeps, sites[0]),
-                                             self._site(ipeps, sites[3]), k % 4, chi,
-                                             self.rsvd_oversampling, self.rsvd_niter,
-                                             self.svd_cutoff, self.sketch.seed, self.sketch.take(),
-                                             precision=self.precision))
+                ipeps[s]  # raises ValueError for an unknown site
+            stream_id = self.sketch.take() if self.svd_type == "rsvd" else 0
+            started.append(_hip.ProjectorJob(slot, [self._site(ipeps, s) for s in sites], k % 4,
+                                             chi, self.rsvd_oversampling, self.rsvd_niter,
+                                             self.svd_cutoff, self.sketch.seed, stream_id,
+                                             precision=self.precision, system=system,
+                                             svd=self.svd_type))
         torch.cuda.synchronize()
         out = []
         for job in started:
@@ -98,6 +87,3 @@ class ProjectorCalculator:
         torch.cuda.synchronize()
         return out
 
-    def calculate_full_system(self, ipeps, sites, k):
-        raise NotImplementedError("projectors='full-system' is not provided by the HIP engine yet; "
-                                  "use 'half-system'")

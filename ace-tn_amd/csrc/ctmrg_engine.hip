@@ -270,6 +270,186 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
 }
 
 // ---------------------------------------------------------------------------
+// The other projector branches of ProjectorCalculator (projectors.py:62-136,
+// :176-217): the full-system rSVD of Q2 Q1 Q4 Q3 (fused_3matmul_svd_lowrank,
+// linalg/fused_3matmul_svd_lowrank.py:4-56, with its QR between the two
+// halves of each power iteration) and the full-rank SVD of the half-system
+// (Q1 Q4) or full-system (Q2 Q1)(Q4 Q3) matrix.  Written against the left
+// operator L = Q1 or Q2 Q1 (a x c) and the right operator R = Q4 or Q4 Q3
+// (c x f): P1 = L^T U S^-1/2, P2 = R V S^-1/2, V = M^T U S^-1 with M = L R.
+// The full-rank branch uses the two-pass Jacobi SVD, so min(a, f) <= 510.
+// ---------------------------------------------------------------------------
+enum { kSysHalf = 0, kSysFull = 1 };
+enum { kSvdRsvd = 0, kSvdFullRank = 1 };
+
+void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, int svd,
+                        int64_t chi, int64_t oversampling, int niter, double cutoff,
+                        uint64_t seed, uint64_t sstream, const HalfOut& out, int phase = 0) {
+  const bool full = system == kSysFull;
+  const acetn_site* s1 = st[0];
+  const acetn_site* s4 = st[3];
+  const QuarterDims d1 = quarter_dims(s1, k);
+  const QuarterDims d4 = quarter_dims(s4, k + 3);
+  ACETN_REQUIRE(d1.N2 == d4.N1, "projectors: Q1 columns != Q4 rows");
+  QuarterDims d2{}, d3{};
+  if (full) {
+    d2 = quarter_dims(st[1], k + 1);
+    d3 = quarter_dims(st[2], k + 2);
+    ACETN_REQUIRE(d2.N2 == d1.N1, "full system: Q2 columns != Q1 rows");
+    ACETN_REQUIRE(d4.N2 == d3.N1, "full system: Q4 columns != Q3 rows");
+  }
+  // L = [Q2] Q1 : a x c ; R = Q4 [Q3] : c x f ; b = Q1 rows, e = Q4 columns
+  const int64_t b = d1.N1, cc = d1.N2, e = d4.N2;
+  const int64_t a = full ? d2.N1 : b, f = full ? d3.N2 : e;
+  int64_t q = (svd == kSvdRsvd) ? std::min<int64_t>(chi + oversampling, std::min(a, f))
+                                : std::min(a, f);
+  ACETN_REQUIRE(q >= 1 && q <= 510, svd == kSvdRsvd
+                                        ? "projectors: sketch width out of range"
+                                        : "projectors: the full-rank SVD on the device needs "
+                                          "min(rows, cols) <= 510 (use svd_type='rsvd')");
+  if (svd == kSvdFullRank) q = a;  // all left singular vectors
+  ACETN_REQUIRE(q <= 510, "projectors: the full-rank SVD on the device needs rows <= 510");
+  double* Q1 = c.take((size_t)b * cc);
+  double* Q4 = c.take((size_t)cc * e);
+  double* Q2 = full ? c.take((size_t)a * b) : nullptr;
+  double* Q3 = full ? c.take((size_t)e * f) : nullptr;
+  const int64_t wid = std::max<int64_t>(q, chi);
+  double* Om = c.take((size_t)f * q);
+  double* Y = c.take((size_t)a * q);
+  double* Z = c.take((size_t)f * q);
+  double* W = c.take((size_t)cc * wid);
+  double* W2 = c.take((size_t)cc * wid);
+  double* tb = c.take((size_t)b * wid);
+  double* te = c.take((size_t)e * wid);
+  double* BT = c.take((size_t)f * q);
+  double* w = c.take((size_t)q);
+  double* UB = c.take((size_t)q * q);
+  double* sv = c.take((size_t)q);
+  int* cD_dev = c.take_int(2);
+  double* X = c.take((size_t)q * wid);
+  double* Vp = c.take((size_t)f * wid);
+  double* Lm = (svd == kSvdFullRank && full) ? c.take((size_t)a * cc) : nullptr;
+  double* Rm = (svd == kSvdFullRank && full) ? c.take((size_t)cc * f) : nullptr;
+
+  const bool replay = c.run && phase == 2;
+  if (replay) c.launch = false;
+  quarter_tensor(c, s1, k, Q1);
+  quarter_tensor(c, s4, k + 3, Q4);
+  if (full) {
+    quarter_tensor(c, st[1], k + 1, Q2);
+    quarter_tensor(c, st[2], k + 2, Q3);
+  }
+  // the four operator applications on n-column blocks
+  auto apply_R = [&](const double* Xin, int n, double* o) {   // f x n -> c x n
+    if (full) {
+      mm(c, (int)e, n, (int)f, false, Q3, f, false, Xin, n, te, n);
+      mm(c, (int)cc, n, (int)e, false, Q4, e, false, te, n, o, n);
+    } else {
+      mm(c, (int)cc, n, (int)e, false, Q4, e, false, Xin, n, o, n);
+    }
+  };
+  auto apply_L = [&](const double* Xin, int n, double* o) {   // c x n -> a x n
+    if (full) {
+      mm(c, (int)b, n, (int)cc, false, Q1, cc, false, Xin, n, tb, n);
+      mm(c, (int)a, n, (int)b, false, Q2, b, false, tb, n, o, n);
+    } else {
+      mm(c, (int)b, n, (int)cc, false, Q1, cc, false, Xin, n, o, n);
+    }
+  };
+  auto apply_LT = [&](const double* Xin, int n, double* o) {  // a x n -> c x n
+    if (full) {
+      mm(c, (int)b, n, (int)a, true, Q2, b, false, Xin, n, tb, n);
+      mm(c, (int)cc, n, (int)b, true, Q1, cc, false, tb, n, o, n);
+    } else {
+      mm(c, (int)cc, n, (int)b, true, Q1, cc, false, Xin, n, o, n);
+    }
+  };
+  auto apply_RT = [&](const double* Xin, int n, double* o) {  // c x n -> f x n
+    if (full) {
+      mm(c, (int)e, n, (int)cc, true, Q4, e, false, Xin, n, te, n);
+      mm(c, (int)f, n, (int)e, true, Q3, f, false, te, n, o, n);
+    } else {
+      mm(c, (int)e, n, (int)cc, true, Q4, e, false, Xin, n, o, n);
+    }
+  };
+  const int qi = (int)q;
+  SplitScope sc(c, 2);
+  if (svd == kSvdRsvd) {
+    if (c.launch) gaussian_sketch(Om, f, q, seed, sstream, c.st);
+    apply_R(Om, qi, W);
+    apply_L(W, qi, Y);
+    for (int it = 0; it < niter; ++it) {
+      if (it >= 1) c.split = c.mixed ? 3 : 0;
+      orthonormalize(c, Y, a, q);
+      apply_LT(Y, qi, W);
+      apply_RT(W, qi, Z);
+      if (full) orthonormalize(c, Z, f, q);
+      apply_R(Z, qi, W2);
+      apply_L(W2, qi, Y);
+    }
+    c.split = c.mixed ? 3 : 0;
+    orthonormalize(c, Y, a, q);
+    apply_LT(Y, qi, W);   // W = L^T Qm, kept for P1
+    apply_RT(W, qi, BT);
+  } else {
+    // BT = (L R)^T = R^T L^T
+    c.split = c.mixed ? 3 : 0;
+    const double* Lp = Q1;
+    const double* Rp = Q4;
+    if (full) {
+      mm(c, (int)a, (int)cc, (int)b, false, Q2, b, false, Q1, cc, Lm, cc);
+      mm(c, (int)cc, (int)f, (int)e, false, Q4, e, false, Q3, f, Rm, f);
+      Lp = Lm;
+      Rp = Rm;
+    }
+    mm(c, (int)f, (int)a, (int)cc, true, Rp, f, true, Lp, cc, BT, a);
+  }
+  small_svd(c, BT, f, q, w, UB);
+  if (!c.run) {
+    const int cDp = (int)std::min<int64_t>(chi, q);
+    for (int cd = 1; cd <= cDp; cd = (cd == cDp) ? cDp + 1 : std::min(2 * cd, cDp)) {
+      mm(c, (int)cc, cd, qi, false, nullptr, q, false, nullptr, cd, nullptr, cd);
+      apply_LT(nullptr, cd, nullptr);
+      mm(c, (int)f, cd, qi, false, nullptr, q, false, nullptr, cd, nullptr, cd);
+      apply_R(nullptr, cd, nullptr);
+    }
+    return;
+  }
+  int cD = 0;
+  if (phase == 2) {
+    c.launch = true;
+    cD = out.cD_in;
+  } else {
+    count_kept(w, qi, (int)chi, cutoff, sv, cD_dev, c.st);
+    if (out.spectrum) ACETN_HIP_CHECK(hipMemcpyAsync(out.spectrum, sv, sizeof(double) * q, hipMemcpyDeviceToDevice, c.st));
+    if (phase == 1) {
+      ACETN_HIP_CHECK(hipMemcpyAsync(out.pinned_cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+      return;
+    }
+    ACETN_HIP_CHECK(hipMemcpyAsync(&cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+    ACETN_HIP_CHECK(hipStreamSynchronize(c.st));
+    *out.host_cD = cD;
+  }
+  ACETN_REQUIRE(cD >= 1 && cD <= std::min<int64_t>(chi, q), "projectors: no singular value above the cutoff");
+  c.split = 0;
+  // P1 = L^T U S^-1/2 (U = Qm UB in the rSVD branch, UB in the full-rank one)
+  scale_columns(UB, qi, sv, cD, 0, X, c.st);
+  if (svd == kSvdRsvd) {
+    mm(c, (int)cc, cD, qi, false, W, q, false, X, cD, out.P1, cD);
+  } else {
+    c.split = c.mixed ? 3 : 0;
+    apply_LT(X, cD, out.P1);
+    c.split = 0;
+  }
+  // P2 = R V S^-1/2, V = B^T UB S^-1
+  scale_columns(UB, qi, sv, cD, 1, X, c.st);
+  mm(c, (int)f, cD, qi, false, BT, q, false, X, cD, Vp, cD);
+  c.split = c.mixed ? 3 : 0;
+  apply_R(Vp, cD, out.P2);
+  fix_projector_gauge(out.P1, out.P2, cc, cc, cD, c.st);
+}
+
+// ---------------------------------------------------------------------------
 // renormalize_boundary (directional_mover.py:273-366)
 // ---------------------------------------------------------------------------
 void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const double* P2i,
@@ -547,6 +727,18 @@ void bond_rdm(Ctx& c, const acetn_site* s1, const acetn_site* s2, int k, double*
        r1, r2, rho);
 }
 
+void check_projector_args(const acetn_site* const* sites, int system, int svd_type) {
+  ACETN_REQUIRE(sites != nullptr, "projectors: null site array");
+  ACETN_REQUIRE(system == kSysHalf || system == kSysFull, "projectors: unknown system");
+  ACETN_REQUIRE(svd_type == kSvdRsvd || svd_type == kSvdFullRank, "projectors: unknown svd type");
+  check_site(sites[0]);
+  check_site(sites[3]);
+  if (system == kSysFull) {
+    check_site(sites[1]);
+    check_site(sites[2]);
+  }
+}
+
 }  // namespace
 }  // namespace acetn
 
@@ -658,6 +850,65 @@ int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, in
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       c.mixed = precision == ACETN_HIP_MIXED;
       half_projectors(c, s1, s4, k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
+    });
+  });
+}
+
+size_t acetn_hip_projectors_workspace(const acetn_site* const sites[4], int k, int64_t chi,
+                                      int64_t oversampling, int system, int svd_type) {
+  size_t r = 0;
+  guarded([&] {
+    check_projector_args(sites, system, svd_type);
+    HalfOut o{nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+    r = measure([&](Ctx& c) {
+      if (system == kSysHalf && svd_type == kSvdRsvd)
+        half_projectors(c, sites[0], sites[3], k, chi, oversampling, 2, 0.0, 0, 0, o);
+      else
+        general_projectors(c, sites, k, system, svd_type, chi, oversampling, 2, 0.0, 0, 0, o);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_projectors_begin(const acetn_site* const sites[4], int k, int64_t chi,
+                               int64_t oversampling, int niter, double svd_cutoff, uint64_t seed,
+                               uint64_t sketch_stream, int system, int svd_type, int precision,
+                               int32_t* pinned_cD_new, double* spectrum, void* ws,
+                               size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_projector_args(sites, system, svd_type);
+    check_precision(precision);
+    ACETN_REQUIRE(pinned_cD_new, "projectors_begin: null cD target");
+    ACETN_REQUIRE(niter >= 0 && niter <= 16, "projectors: niter out of range");
+    HalfOut o{nullptr, nullptr, nullptr, pinned_cD_new, 0, spectrum};
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
+      if (system == kSysHalf && svd_type == kSvdRsvd)
+        half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, svd_cutoff, seed,
+                        sketch_stream, o, 1);
+      else
+        general_projectors(c, sites, k, system, svd_type, chi, oversampling, niter, svd_cutoff,
+                           seed, sketch_stream, o, 1);
+    });
+  });
+}
+
+int acetn_hip_projectors_end(const acetn_site* const sites[4], int k, int64_t chi,
+                             int64_t oversampling, int niter, int system, int svd_type,
+                             int precision, int64_t cD_new, double* P1, double* P2, void* ws,
+                             size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_projector_args(sites, system, svd_type);
+    check_precision(precision);
+    ACETN_REQUIRE(P1 && P2, "projectors_end: null output");
+    HalfOut o{P1, P2, nullptr, nullptr, (int)cD_new, nullptr};
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
+      if (system == kSysHalf && svd_type == kSvdRsvd)
+        half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
+      else
+        general_projectors(c, sites, k, system, svd_type, chi, oversampling, niter, 0.0, 0, 0, o,
+                           2);
     });
   });
 }

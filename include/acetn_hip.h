@@ -119,6 +119,31 @@ int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, in
                                   double* P1, double* P2, void* workspace, size_t ws_bytes,
                                   void* stream);
 
+/* All projector branches of ProjectorCalculator (projectors.py:21-236):
+ * system ACETN_HIP_HALF_SYSTEM (calculate_half_system, :138-174; sites[0] and
+ * sites[3] are used) or ACETN_HIP_FULL_SYSTEM (calculate_full_system,
+ * :176-217, with fused_3matmul_svd_lowrank, linalg/fused_3matmul_svd_lowrank.py:4-56);
+ * svd_type ACETN_HIP_RSVD (randomised, the default) or ACETN_HIP_FULL_RANK
+ * (contract_half_system/contract_full_system + svd, :62-136; on the device
+ * for matrices with at most 510 rows).  sites[0..3] are the plaquette s1..s4
+ * of acetn/renormalization/directional_mover.py:99-181.  Split at the host
+ * readback of cD_new like acetn_hip_half_projectors_begin/_end; the
+ * full-rank branch draws no sketch.  P1, P2: (rows x chi) buffers written
+ * as rows x cD_new. */
+enum { ACETN_HIP_HALF_SYSTEM = 0, ACETN_HIP_FULL_SYSTEM = 1 };
+enum { ACETN_HIP_RSVD = 0, ACETN_HIP_FULL_RANK = 1 };
+size_t acetn_hip_projectors_workspace(const acetn_site* const sites[4], int k, int64_t chi,
+                                      int64_t oversampling, int system, int svd_type);
+int acetn_hip_projectors_begin(const acetn_site* const sites[4], int k, int64_t chi,
+                               int64_t oversampling, int niter, double svd_cutoff, uint64_t seed,
+                               uint64_t sketch_stream, int system, int svd_type, int precision,
+                               int32_t* pinned_cD_new, double* spectrum, void* workspace,
+                               size_t ws_bytes, void* stream);
+int acetn_hip_projectors_end(const acetn_site* const sites[4], int k, int64_t chi,
+                             int64_t oversampling, int niter, int system, int svd_type,
+                             int precision, int64_t cD_new, double* P1, double* P2,
+                             void* workspace, size_t ws_bytes, void* stream);
+
 /* Boundary renormalisation of one (s1 -> s2) target.
  * Replaces DirectionalMover.renormalize_boundary / renormalize_cj1 /
  * renormalize_cj2 / renormalize_ej, acetn/renormalization/directional_mover.py:273-366.

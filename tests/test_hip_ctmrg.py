@@ -188,3 +188,34 @@ def test_mixed_ctmrg_on_golden_fixture(hip, case):
     e_h = om.measure(hst.to_numpy(), 2, 2, model)['Energy']
     print(f"{case}: mixed {e_h:.15f} golden {e_ref:.15f} dE {abs(e_h - e_ref):.2e}")
     assert abs(e_h - e_ref) <= MIXED_DE
+
+
+@pytest.mark.parametrize("projectors,svd_type,D,chi", [("full-system", "rsvd", 2, 8),
+                                                        ("full-system", "rsvd", 3, 12),
+                                                        ("half-system", "full-rank", 2, 8),
+                                                        ("full-system", "full-rank", 2, 8)])
+def test_other_projector_branches_match_oracle(hip, projectors, svd_type, D, chi):
+    """Full-system rSVD (fused_3matmul_svd_lowrank) and full-rank SVD
+    projectors (projectors.py:62-136, :176-217): three CTMRG iterations
+    against the oracle's same branch."""
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+
+    class BCfg(Cfg):
+        pass
+    BCfg.projectors, BCfg.svd_type = projectors, svd_type
+    st0 = oc.make_state(oc.random_site_tensors(2, 2, D, 2, seed=500 + chi, bias=0.5))
+    seed = 31
+    ost = copy.deepcopy(st0)
+    ocalc = oc.ProjectorCalculator(projectors=projectors, svd_type=svd_type,
+                                   omega_fn=SketchStream(seed))
+    omover = oc.DirectionalMover(ocalc, 2, 2, chi)
+    hst = DevState(st0, 2, 2, D, chi, seed)
+    hmover = DirectionalMover(BCfg(), hst.sketch)
+    for _ in range(3):
+        oc.ctmrg_iteration(omover, ost)
+        ctmrg_iteration(hmover, hst)
+    assert_state_close(hst.to_numpy(), ost, rtol=1e-8)
+    e_o = om.measure(ost, 2, 2, omod.Heisenberg(1.0))['Energy']
+    e_h = om.measure(hst.to_numpy(), 2, 2, omod.Heisenberg(1.0))['Energy']
+    assert e_h == pytest.approx(e_o, rel=1e-10)

@@ -124,6 +124,16 @@ def _declare(lib):
         "acetn_hip_simple_update_bond": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp * 4,
                                                         c_vp * 4, c_vp, c_i64, c_i64, c_dbl, c_vp,
                                                         c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_mlp_param_count": (c_i64, [c_i64, c_i64, c_i64]),
+        "acetn_hip_mlp_workspace": (c_size, [c_i64, c_i64, c_i64, c_i64]),
+        "acetn_hip_mlp_forward": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp,
+                                                 c_vp, c_size, c_vp]),
+        "acetn_hip_mlp_backward": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp,
+                                                  c_vp, c_vp, ctypes.POINTER(ctypes.c_float),
+                                                  c_vp, c_size, c_vp]),
+        "acetn_hip_mlp_adam_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
+                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                                   ctypes.c_float, ctypes.c_float, c_vp]),
         "acetn_hip_eigh_workspace": (c_size, [c_i64]),
         "acetn_hip_eigh": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_orthonormalize_workspace": (c_size, [c_i64, c_i64]),
@@ -594,3 +604,39 @@ def simple_update_bond(A1, A2, k, lam1, lam2, gate, cutoff=1e-12):
                                              stream_handle()))
     del keep
     return A1o, A2o, lam
+
+
+# ------------------------------------------------------------ learned ALS start
+def mlp_param_count(fin, hidden, fout):
+    return int(lib().acetn_hip_mlp_param_count(fin, hidden, fout))
+
+
+def mlp_forward(params, fin, hidden, fout, X):
+    """Y = MLP(X) for X (batch x fin) FP32 on the device."""
+    X = X.contiguous()
+    B = X.shape[0]
+    Y = torch.empty(B, fout, dtype=torch.float32, device=X.device)
+    ws = workspace(lib().acetn_hip_mlp_workspace(fin, hidden, fout, B), X.device)
+    check(lib().acetn_hip_mlp_forward(ptr(params), fin, hidden, fout, ptr(X), B, ptr(Y), ptr(ws),
+                                      ws.numel(), stream_handle()))
+    return Y
+
+
+def mlp_backward(params, fin, hidden, fout, X, T, grad):
+    """Forward + MSE loss + backward; fills ``grad``; returns (loss, dL/dY)."""
+    X, T = X.contiguous(), T.contiguous()
+    B = X.shape[0]
+    Y = torch.empty(B, fout, dtype=torch.float32, device=X.device)
+    loss = ctypes.c_float(0.0)
+    ws = workspace(lib().acetn_hip_mlp_workspace(fin, hidden, fout, B), X.device)
+    check(lib().acetn_hip_mlp_backward(ptr(params), fin, hidden, fout, ptr(X), B, ptr(T), ptr(Y),
+                                       ptr(grad), ctypes.byref(loss), ptr(ws), ws.numel(),
+                                       stream_handle()))
+    return loss.value, Y
+
+
+def mlp_adam_step(params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8,
+                  weight_decay=0.0):
+    check(lib().acetn_hip_mlp_adam_step(ptr(params), ptr(grad), ptr(m), ptr(v), params.numel(),
+                                        int(step), lr, beta1, beta2, eps, weight_decay,
+                                        stream_handle()))

@@ -257,6 +257,28 @@ int acetn_hip_simple_update_bond(const double* A1, const double* A2, int k,
                                  double* A1_out, double* A2_out, double* lam_out, void* workspace,
                                  size_t ws_bytes, void* stream);
 
+/* ---- learned ALS start (no reference counterpart) ------------------------
+ * A 3-layer perceptron Fin -> hidden -> hidden -> Fout with ReLU, weights in
+ * one flat FP32 buffer [W1 (Fin x H), b1, W2 (H x H), b2, W3 (H x Fout), b3]
+ * (matrices row-major [in][out]); every product runs on the bf16 matrix
+ * cores with FP32 accumulation.  X (batch x Fin) and Y (batch x Fout) are
+ * FP32 row-major.  _backward runs the forward pass, the mean-squared loss
+ * against target (written to host_loss), and writes the gradient (same
+ * layout as params) and dL/dY over Y.  _adam_step is Adam with decoupled
+ * bias correction (step >= 1) and L2 weight decay folded into the gradient. */
+int64_t acetn_hip_mlp_param_count(int64_t fin, int64_t hidden, int64_t fout);
+size_t acetn_hip_mlp_workspace(int64_t fin, int64_t hidden, int64_t fout, int64_t batch);
+int acetn_hip_mlp_forward(const float* params, int64_t fin, int64_t hidden, int64_t fout,
+                          const float* X, int64_t batch, float* Y, void* workspace,
+                          size_t ws_bytes, void* stream);
+int acetn_hip_mlp_backward(const float* params, int64_t fin, int64_t hidden, int64_t fout,
+                           const float* X, int64_t batch, const float* target, float* Y,
+                           float* grad, float* host_loss, void* workspace, size_t ws_bytes,
+                           void* stream);
+int acetn_hip_mlp_adam_step(float* params, const float* grad, float* m, float* v, int64_t n,
+                            int64_t step, float lr, float beta1, float beta2, float eps,
+                            float weight_decay, void* stream);
+
 /* ---- building blocks, exported for the parity tests ---------------------- */
 
 /* Gaussian sketch, n x q row-major (the device side of the generator the

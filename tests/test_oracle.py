@@ -228,3 +228,31 @@ def test_simple_update_identity_gate_keeps_bond_product():
     ofu.simple_update_bond(st, b, np.eye(4).reshape(2, 2, 2, 2), lam)
     after = np.einsum("lUxdp,xuRDq->lUdpuRDq", st[b[0]]['A'], st[b[1]]['A'])
     np.testing.assert_allclose(after, before / np.linalg.norm(S), atol=1e-12)
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The perceptron oracle's backward pass (FP64, no bf16 rounding) against
+    central differences of its own loss."""
+    from oracle import mlp as omlp
+    rng = np.random.default_rng(4)
+    fin, h, fout, B = 7, 5, 3, 6
+    n = fin * h + h + h * h + h + h * fout + fout
+    p = rng.standard_normal(n)
+    X, T = rng.standard_normal((B, fin)), rng.standard_normal((B, fout))
+    _, g = omlp.loss_and_grad(p, fin, h, fout, X, T, emulate_bf16=False)
+    eps = 1e-6
+    for i in rng.choice(n, 25, replace=False):
+        e = np.zeros(n); e[i] = eps
+        lp, _ = omlp.loss_and_grad(p + e, fin, h, fout, X, T, emulate_bf16=False)
+        lm, _ = omlp.loss_and_grad(p - e, fin, h, fout, X, T, emulate_bf16=False)
+        assert g[i] == pytest.approx((lp - lm) / (2 * eps), rel=1e-5, abs=1e-8)
+
+
+def test_bf16_rounding_known_values():
+    from oracle import mlp as omlp
+    # bf16 keeps 8 significant bits; ties go to even
+    assert omlp.bf16(1.0) == 1.0
+    assert omlp.bf16(1.0 + 2 ** -8) == 1.0            # tie -> even (down)
+    assert omlp.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6   # tie -> even (up)
+    assert omlp.bf16(1.0 + 2 ** -7) == 1.0 + 2 ** -7
+    assert omlp.bf16(-3.0) == -3.0

@@ -178,6 +178,74 @@ def run_ctmrg(precision, args):
     return {"its": its, "dt": dt, "stats": stats, "energy": energy}
 
 
+def run_als(args):
+    """The full-update ALS workload: Heisenberg J=1 on the 2x2 cell, d=2,
+    D=6, chi=72, gate exp(-0.01 H).  Seeded N(0,1) site tensors, CTMRG
+    environment, then fast-full-update sweeps that record every bond's ALS
+    problem (gauge-fixed n12, a12g) and converged solution.  The perceptron is
+    trained on the first 3/4 of the problems (device kernels) and both starts
+    are run on the rest to relative cost change < 1e-10."""
+    import torch
+    from acetn.evolution.learned_start import (MLP, MLPProposer, dataset, evaluate,
+                                               feature_sizes, generate_problems, solve_from, train)
+    from acetn.ipeps import Ipeps
+    from acetn.model.model_factory import model_factory
+    from acetn import _hip
+    D, chi = 6, 72
+    config = {'dtype': 'float64', 'device': 'cuda',
+              'TN': {'nx': 2, 'ny': 2, 'dims': {'phys': 2, 'bond': D, 'chi': chi}},
+              'model': {'name': 'heisenberg', 'params': {'J': 1.0}},
+              'ctmrg': {'steps': 3, 'sketch_seed': args.seed, 'disable_progressbar': True},
+              'evolution': {'disable_progressbar': True}}
+    ipeps = Ipeps(config)
+    g = torch.Generator().manual_seed(args.seed)
+    for s in ipeps.site_list:
+        a = torch.randn(D, D, D, D, 2, generator=g, dtype=torch.float64)
+        ipeps[s]['A'] = a / a.norm()
+        ipeps[s].initialize_corner_tensors()
+        ipeps[s].initialize_edge_tensors()
+    ipeps.site_states_initialized = True
+    ipeps.renormalize()
+    model = model_factory.create(ipeps.config.model)
+    t0 = time.perf_counter()
+    probs = generate_problems(ipeps, model, 0.01, args.als_steps, ipeps.config.evolution)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    ntr = (3 * len(probs)) // 4
+    train_p, test_p = probs[:ntr], probs[ntr:]
+    fin, fout = feature_sizes(2 * D, 2)
+    X, T = dataset(train_p)
+    mlp = MLP(fin, fout, hidden=512, seed=0)
+    t0 = time.perf_counter()
+    losses = train(mlp, X, T, epochs=args.als_epochs, batch=32, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    cfg = ipeps.config.evolution
+    res = evaluate(MLPProposer(mlp, cfg), test_p, cfg, tol=1e-10)
+    # the reference's default stopping tolerance (als_tol = 1e-15)
+    ref_tol = evaluate(MLPProposer(mlp, cfg), test_p, cfg, tol=cfg.als_tol)
+    res["at_default_tol"] = {"tol": cfg.als_tol, **ref_tol,
+                             "sweep_reduction": 1.0 - ref_tol["mean_sweeps_learned"] /
+                             ref_tol["mean_sweeps_plain"]}
+    # plain ALS solves/s (plain start included), to rel. cost change < 1e-10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for pr in test_p:
+        a1, a2 = _hip.als_initialize(pr['a12g'], D)
+        solve_from(pr['n12'], pr['a12g'], a1, a2, cfg, 1e-10)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res.update({"als_solves_per_s": len(test_p) / dt, "train_problems": ntr,
+                "train_loss_first_last": [losses[0], losses[-1]],
+                "sweep_reduction": 1.0 - res["mean_sweeps_learned"] / res["mean_sweeps_plain"],
+                "generate_s": t_gen, "train_s": t_train,
+                "workload": f"Heisenberg 2x2, D={D}, chi={chi}, dtau=0.01, {args.als_steps} "
+                            f"fast-full-update sweeps from seeded N(0,1) tensors; MLP 3 layers, "
+                            f"width 512, bf16 MFMA, {args.als_epochs} epochs"})
+    log(f"[als] {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +255,10 @@ def main():
     ap.add_argument("--chi", type=int, default=144)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--als-steps", type=int, default=64,
+                    help="fast-full-update sweeps recording ALS problems (8 per sweep)")
+    ap.add_argument("--als-epochs", type=int, default=60)
+    ap.add_argument("--no-als", action="store_true")
     ap.add_argument("--serial", action="store_true",
                     help="run the timed iterations with the projector pairs serialised")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
@@ -258,6 +330,8 @@ def main():
                      "avg_launch_ms": st["ms"] / st["launches"]},
         "kernels": {"mixed": mix["stats"], "fp64": ref["stats"]},
     }
+    if not args.no_als:
+        out["als"] = run_als(args)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(D, chi)

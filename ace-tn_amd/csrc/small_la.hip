@@ -178,6 +178,149 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
   }
 }
 
+// Register-resident variant for q <= kCrMaxQ: the same factor and inverse,
+// with the upper triangle held in registers.  Thread t owns a run of E
+// consecutive rows i0 <= i < i0 + E of one column k (i <= k), so a step reads
+// one value for the column and E consecutive values for the rows.  Both
+// phases are right-looking with one barrier per step:
+//   Cholesky, step j:  row j (raw) is published in a double-buffered row,
+//                      R[j][k] = S[j][k] / sqrt(S[j][j]) goes to the packed
+//                      factor in LDS, S[i][k] -= S[j][i] S[j][k] / S[j][j]
+//                      for j < i <= k in registers;
+//   inverse, step r (descending):  X[r][c] = T[r][c] / R[r][r] is published,
+//                      T[i][c] -= R[i][r] X[r][c] for i < r <= c,
+// with T = I initially, so X = R^-1 row by row.  Dead pivots give zero rows
+// of R and zero rows and columns of X, as in chol_rinv_kernel.
+constexpr int CR_NT = 1024;
+constexpr int kCrMaxQ = 173;  // packed factor (<= 120 KB) in LDS, runs of <= 16 rows
+
+// column runs of length E covering the upper triangle of a q x q matrix
+int cr_runs(int q, int E) {
+  int n = 0;
+  for (int k = 0; k < q; ++k) n += (k + E) / E;
+  return n;
+}
+
+template <int E>
+__global__ __launch_bounds__(CR_NT) void chol_rinv_reg_kernel(const double* __restrict__ G,
+                                                              double* __restrict__ Rinv, int q,
+                                                              double shift_rel, double rank_tol,
+                                                              int* dead_out,
+                                                              const int* only_if_dead) {
+  extern __shared__ double RS[];  // packed upper factor, pk(i, k)
+  __shared__ double buf[2][kCrMaxQ + 4];
+  __shared__ int dead[kCrMaxQ + 4];
+  __shared__ double s_red[CR_NT / 64][2];
+  __shared__ int any;
+  const int tid = threadIdx.x;
+  if (only_if_dead) {
+    if (tid == 0) any = 0;
+    __syncthreads();
+    for (int i = tid; i < q; i += CR_NT)
+      if (only_if_dead[i]) any = 1;
+    __syncthreads();
+    if (!any) {
+      for (int e = tid; e < q * q; e += CR_NT) Rinv[e] = (e / q == e % q) ? 1.0 : 0.0;
+      return;
+    }
+  }
+  double tr = 0.0, dm = 0.0;
+  for (int i = tid; i < q; i += CR_NT) {
+    const double gii = G[(int64_t)i * q + i];
+    tr += gii;
+    dm = fmax(dm, gii);
+  }
+  for (int o = 32; o > 0; o >>= 1) { tr += __shfl_down(tr, o); dm = fmax(dm, __shfl_down(dm, o)); }
+  if ((tid & 63) == 0) { s_red[tid >> 6][0] = tr; s_red[tid >> 6][1] = dm; }
+  __syncthreads();
+  if (tid == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int w = 0; w < CR_NT / 64; ++w) { t0 += s_red[w][0]; t1 = fmax(t1, s_red[w][1]); }
+    s_red[0][0] = t0; s_red[0][1] = t1;
+  }
+  __syncthreads();
+  const double shift = shift_rel * s_red[0][0];
+  const double thresh = rank_tol * (s_red[0][1] + shift);
+
+  // this thread's run: column kc, rows i0 .. min(i0 + E, kc + 1) - 1 (kc = -1: none)
+  int kc = -1, i0 = 0;
+  {
+    int acc = 0;
+    for (int k = 0; k < q; ++k) {
+      const int cnt = (k + E) / E;
+      if (tid < acc + cnt) { kc = k; i0 = (tid - acc) * E; break; }
+      acc += cnt;
+    }
+  }
+  const int iend = kc + 1;  // rows i < iend are in the triangle
+  double v[E];
+#pragma unroll
+  for (int t = 0; t < E; ++t) {
+    const int i = i0 + t;
+    v[t] = (i < iend) ? G[(int64_t)i * q + kc] + (i == kc ? shift : 0.0) : 0.0;
+  }
+  const int colbase = kc * (kc + 1) / 2;  // pk(i, kc) = colbase + i
+
+  for (int j = 0; j < q; ++j) {
+    double* rb = buf[j & 1];
+#pragma unroll
+    for (int t = 0; t < E; ++t)
+      if (i0 + t == j && j < iend) rb[kc] = v[t];
+    __syncthreads();
+    const double d = rb[j];
+    const bool is_dead = !(d > thresh) || !isfinite(d);
+    if (tid == 0) dead[j] = is_dead ? 1 : 0;
+    if (kc < j) continue;  // column finished (uniform per thread)
+    if (is_dead) {
+      if (j >= i0 && j < i0 + E) RS[colbase + j] = 0.0;
+      continue;
+    }
+    const double rk = rb[kc] * (1.0 / d);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int i = i0 + t;
+      if (i == j) RS[colbase + j] = (kc == j) ? sqrt(d) : v[t] / sqrt(d);
+      else if (i > j && i < iend) v[t] -= rb[i] * rk;
+    }
+  }
+  __syncthreads();
+  if (dead_out) {
+    int anyd = 0;
+    for (int i = tid; i < q; i += CR_NT) {
+      dead_out[i] = dead[i];
+      anyd |= dead[i];
+    }
+    anyd = __syncthreads_or(anyd);
+    if (tid == 0) dead_out[q] = anyd;
+  }
+
+#pragma unroll
+  for (int t = 0; t < E; ++t) v[t] = (i0 + t == kc) ? 1.0 : 0.0;
+  for (int r = q - 1; r >= 0; --r) {
+    double* xb = buf[r & 1];
+    const bool dr = dead[r] != 0;
+    const double inv = dr ? 0.0 : 1.0 / RS[r * (r + 1) / 2 + r];
+#pragma unroll
+    for (int t = 0; t < E; ++t)
+      if (i0 + t == r && r < iend) {
+        v[t] *= inv;
+        xb[kc] = v[t];
+      }
+    __syncthreads();
+    if (dr || kc < r) continue;
+    const double x = xb[kc];
+    const double* rcol = RS + r * (r + 1) / 2;  // R[i][r], i <= r
+#pragma unroll
+    for (int t = 0; t < E; ++t)
+      if (i0 + t < r) v[t] -= rcol[i0 + t] * x;
+  }
+#pragma unroll
+  for (int t = 0; t < E; ++t)
+    if (i0 + t < iend) Rinv[(int64_t)(i0 + t) * q + kc] = v[t];
+  for (int e = tid; e < q * q; e += CR_NT)
+    if (e / q > e % q) Rinv[e] = 0.0;
+}
+
 // Dead (numerically rank-deficient) columns of an orthonormalisation are
 // replaced by Gaussian vectors, which the following Cholesky-QR passes
 // orthogonalise against the live ones: like Householder QR, the result always
@@ -537,6 +680,20 @@ void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double ra
                hipStream_t s, double* gscratch, int* dead_out, const int* only_if_dead) {
   ACETN_REQUIRE(q >= 1 && q <= 512, "chol_rinv: q out of range");
   const size_t pb = packed_bytes(q);
+  if (q <= kCrMaxQ) {
+    ACETN_REQUIRE(cr_runs(q, 16) <= CR_NT, "chol_rinv: register layout too small");
+    if (cr_runs(q, 8) <= CR_NT)
+      hipLaunchKernelGGL(chol_rinv_reg_kernel<8>, dim3(1), dim3(CR_NT), pb, s, G, Rinv, q,
+                         shift_rel, rank_tol, dead_out, only_if_dead);
+    else if (cr_runs(q, 12) <= CR_NT)
+      hipLaunchKernelGGL(chol_rinv_reg_kernel<12>, dim3(1), dim3(CR_NT), pb, s, G, Rinv, q,
+                         shift_rel, rank_tol, dead_out, only_if_dead);
+    else
+      hipLaunchKernelGGL(chol_rinv_reg_kernel<16>, dim3(1), dim3(CR_NT), pb, s, G, Rinv, q,
+                         shift_rel, rank_tol, dead_out, only_if_dead);
+    ACETN_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const int use_lds = pb <= kLdsBudget;
   ACETN_REQUIRE(use_lds || gscratch != nullptr, "chol_rinv: scratch required");
   if (use_lds)

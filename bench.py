@@ -171,6 +171,12 @@ def run_ctmrg(precision, args):
             stats[name] = {"ms": ms, "flops": fl, "launches": n}
     _hip.kernel_timing(False)
     energy = heisenberg_energy(net)
+    spec = mover.projector_calculator.last_spectrum
+    if spec is not None:
+        sv = spec.cpu()
+        sv = sv[sv > 0]
+        log(f"[{precision}] last projector spectrum: s_min/s_0 = {float(sv[-1] / sv[0]):.3e} "
+            f"over {len(sv)} values")
     its = args.steps / dt
     log(f"[{precision}] {its:.3f} it/s, E/site {energy:.12f}; " +
         "; ".join(f"{k}: {v['launches']} launches {v['ms']:.1f} ms "

@@ -110,7 +110,9 @@ def test_eigh_graded_and_degenerate(hip):
 
 
 @pytest.mark.parametrize("m,q,rank", [(200, 20, 20), (1024, 66, 66), (300, 30, 1), (80, 22, 16),
-                                      (16, 16, 12)])
+                                      (16, 16, 12), (600, 100, 100), (2000, 146, 146),
+                                      (1500, 146, 60), (2000, 170, 170), (900, 173, 150),
+                                      (1200, 174, 174), (1200, 220, 200)])
 def test_orthonormalize(hip, m, q, rank):
     # q orthonormal columns always (rank-deficient sketches are completed, as
     # Householder QR does), and their span contains range(Y)

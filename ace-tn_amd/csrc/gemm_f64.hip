@@ -241,7 +241,9 @@ struct Tile {
 // per-shape tile: skinny range-finder products (N <= 160) use 64 x 160 / 32 x 160,
 // large products 128 x 128, small ones 64 x 64
 Tile choose_tile(const GemmDesc& g) {
-  if (g.N > 96 && g.N <= 160) return g.M >= 2048 ? Tile{2, 5} : Tile{1, 5};
+  // skinny products: 64-row tiles only when K is long enough to split
+  // (short-K products such as Y R^-1 need the tile count)
+  if (g.N > 96 && g.N <= 160) return (g.M >= 2048 && g.K >= 1024) ? Tile{2, 5} : Tile{1, 5};
   if (g.M >= 1024 && g.N >= 1024) return Tile{4, 4};
   return Tile{2, 2};
 }

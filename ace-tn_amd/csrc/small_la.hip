@@ -179,40 +179,78 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
 }
 
 // Register-resident variant for q <= kCrMaxQ: the same factor and inverse,
-// with the upper triangle held in registers.  Thread t owns a run of E
-// consecutive rows i0 <= i < i0 + E of one column k (i <= k), so a step reads
-// one value for the column and E consecutive values for the rows.  Both
-// phases are right-looking with one barrier per step:
+// with the upper triangle held in registers.  Wave w owns the rows
+// i = w + CR_WAVES r and lane l the columns k = i + l + 64 s of each (k >= i),
+// so a step reads the published row with consecutive addresses per wave (no
+// LDS bank conflicts).  The rows a wave still updates form a suffix (Cholesky)
+// or a prefix (inverse) of its row slots, entered through one switch, and
+// every column slot is updated unconditionally (the row buffers are zero past
+// q), so a step costs little scalar control flow.  Both phases are
+// right-looking with one barrier per step:
 //   Cholesky, step j:  row j (raw) is published in a double-buffered row,
 //                      R[j][k] = S[j][k] / sqrt(S[j][j]) goes to the packed
-//                      factor in LDS, S[i][k] -= S[j][i] S[j][k] / S[j][j]
+//                      factor in LDS, S[i][k] -= (S[j][i] / S[j][j]) S[j][k]
 //                      for j < i <= k in registers;
-//   inverse, step r (descending):  X[r][c] = T[r][c] / R[r][r] is published,
-//                      T[i][c] -= R[i][r] X[r][c] for i < r <= c,
+//   inverse, step r (descending):  X[r][c] = T[r][c] / R[r][r] is published
+//                      (entries c < r of the buffer stay zero),
+//                      T[i][c] -= R[i][r] X[r][c] for i < r,
 // with T = I initially, so X = R^-1 row by row.  Dead pivots give zero rows
 // of R and zero rows and columns of X, as in chol_rinv_kernel.
-constexpr int CR_NT = 1024;
-constexpr int kCrMaxQ = 173;  // packed factor (<= 120 KB) in LDS, runs of <= 16 rows
+constexpr int CR_WAVES = 8;
+constexpr int CR_NT = 64 * CR_WAVES;
+constexpr int kCrMaxQ = 173;                                   // packed factor (<= 120 KB) in LDS
+constexpr int CR_ROWS = (kCrMaxQ + CR_WAVES - 1) / CR_WAVES;   // row slots per wave (<= 24)
+constexpr int CR_SLOTS = (kCrMaxQ + 63) / 64;                  // column slots per row
+constexpr int kCrBuf = kCrMaxQ + 64 * CR_SLOTS;                // row buffers, zero past q
+// row slot r holds rows i >= CR_WAVES r, so it needs at most cr_slots(r) column slots
+__host__ __device__ constexpr int cr_slots(int r) { return (kCrMaxQ - CR_WAVES * r + 63) / 64; }
+__host__ __device__ constexpr int cr_off(int r) { return r == 0 ? 0 : cr_off(r - 1) + cr_slots(r - 1); }
+constexpr int CR_REGS = cr_off(CR_ROWS);
+static_assert(CR_ROWS <= 24, "chol_rinv_reg: switch covers 24 row slots");
 
-// column runs of length E covering the upper triangle of a q x q matrix
-int cr_runs(int q, int E) {
-  int n = 0;
-  for (int k = 0; k < q; ++k) n += (k + E) / E;
-  return n;
+// v[row slot R] -= a * buf[i + lane + 64 s] for every column slot
+template <int R>
+__device__ __forceinline__ void cr_update(double (&v)[CR_REGS], double a, const double* b) {
+  if constexpr (R < CR_ROWS) {
+#pragma unroll
+    for (int sl = 0; sl < cr_slots(R); ++sl) v[cr_off(R) + sl] -= a * b[64 * sl];
+  }
+}
+// copy row slot R to/from the row buffer (columns < q only), scaling by f
+template <int R>
+__device__ __forceinline__ void cr_publish(double (&v)[CR_REGS], double f, double* b, int k0,
+                                           int q) {
+  if constexpr (R < CR_ROWS) {
+#pragma unroll
+    for (int sl = 0; sl < cr_slots(R); ++sl) {
+      v[cr_off(R) + sl] *= f;
+      if (k0 + 64 * sl < q) b[64 * sl] = v[cr_off(R) + sl];
+    }
+  }
+}
+template <int R>
+__device__ __forceinline__ void cr_factor_row(const double (&v)[CR_REGS], double rjj, double irjj,
+                                              double* RS, int j, int lane, int q) {
+  if constexpr (R < CR_ROWS) {
+#pragma unroll
+    for (int sl = 0; sl < cr_slots(R); ++sl) {
+      const int k = j + lane + 64 * sl;
+      if (k < q) RS[pk(j, k)] = (k == j) ? rjj : v[cr_off(R) + sl] * irjj;
+    }
+  }
 }
 
-template <int E>
 __global__ __launch_bounds__(CR_NT) void chol_rinv_reg_kernel(const double* __restrict__ G,
                                                               double* __restrict__ Rinv, int q,
                                                               double shift_rel, double rank_tol,
                                                               int* dead_out,
                                                               const int* only_if_dead) {
   extern __shared__ double RS[];  // packed upper factor, pk(i, k)
-  __shared__ double buf[2][kCrMaxQ + 4];
+  __shared__ double buf[2][kCrBuf];
   __shared__ int dead[kCrMaxQ + 4];
-  __shared__ double s_red[CR_NT / 64][2];
+  __shared__ double s_red[CR_WAVES][2];
   __shared__ int any;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (only_if_dead) {
     if (tid == 0) any = 0;
     __syncthreads();
@@ -224,6 +262,7 @@ __global__ __launch_bounds__(CR_NT) void chol_rinv_reg_kernel(const double* __re
       return;
     }
   }
+  for (int e = tid; e < 2 * kCrBuf; e += CR_NT) (&buf[0][0])[e] = 0.0;
   double tr = 0.0, dm = 0.0;
   for (int i = tid; i < q; i += CR_NT) {
     const double gii = G[(int64_t)i * q + i];
@@ -231,56 +270,141 @@ __global__ __launch_bounds__(CR_NT) void chol_rinv_reg_kernel(const double* __re
     dm = fmax(dm, gii);
   }
   for (int o = 32; o > 0; o >>= 1) { tr += __shfl_down(tr, o); dm = fmax(dm, __shfl_down(dm, o)); }
-  if ((tid & 63) == 0) { s_red[tid >> 6][0] = tr; s_red[tid >> 6][1] = dm; }
+  if (lane == 0) { s_red[wave][0] = tr; s_red[wave][1] = dm; }
   __syncthreads();
   if (tid == 0) {
     double t0 = 0.0, t1 = 0.0;
-    for (int w = 0; w < CR_NT / 64; ++w) { t0 += s_red[w][0]; t1 = fmax(t1, s_red[w][1]); }
+    for (int w = 0; w < CR_WAVES; ++w) { t0 += s_red[w][0]; t1 = fmax(t1, s_red[w][1]); }
     s_red[0][0] = t0; s_red[0][1] = t1;
   }
   __syncthreads();
   const double shift = shift_rel * s_red[0][0];
   const double thresh = rank_tol * (s_red[0][1] + shift);
 
-  // this thread's run: column kc, rows i0 .. min(i0 + E, kc + 1) - 1 (kc = -1: none)
-  int kc = -1, i0 = 0;
-  {
-    int acc = 0;
-    for (int k = 0; k < q; ++k) {
-      const int cnt = (k + E) / E;
-      if (tid < acc + cnt) { kc = k; i0 = (tid - acc) * E; break; }
-      acc += cnt;
+  // v[cr_off(r) + s] = entry (i, k), i = wave + CR_WAVES r, k = i + lane + 64 s
+  double v[CR_REGS];
+#pragma unroll
+  for (int r = 0; r < CR_ROWS; ++r) {
+    const int i = wave + CR_WAVES * r;
+#pragma unroll
+    for (int sl = 0; sl < cr_slots(r); ++sl) {
+      const int k = i + lane + 64 * sl;
+      v[cr_off(r) + sl] = (i < q && k < q) ? G[(int64_t)i * q + k] + (k == i ? shift : 0.0) : 0.0;
     }
   }
-  const int iend = kc + 1;  // rows i < iend are in the triangle
-  double v[E];
-#pragma unroll
-  for (int t = 0; t < E; ++t) {
-    const int i = i0 + t;
-    v[t] = (i < iend) ? G[(int64_t)i * q + kc] + (i == kc ? shift : 0.0) : 0.0;
-  }
-  const int colbase = kc * (kc + 1) / 2;  // pk(i, kc) = colbase + i
 
   for (int j = 0; j < q; ++j) {
     double* rb = buf[j & 1];
-#pragma unroll
-    for (int t = 0; t < E; ++t)
-      if (i0 + t == j && j < iend) rb[kc] = v[t];
+    const int jr = j / CR_WAVES;
+    const bool owner = (j % CR_WAVES) == wave;
+    if (owner) {
+      switch (jr) {
+#define CR_CASE(R) \
+  case R: cr_publish<R>(v, 1.0, rb + j + lane, j + lane, q); break;
+    CR_CASE(0)
+    CR_CASE(1)
+    CR_CASE(2)
+    CR_CASE(3)
+    CR_CASE(4)
+    CR_CASE(5)
+    CR_CASE(6)
+    CR_CASE(7)
+    CR_CASE(8)
+    CR_CASE(9)
+    CR_CASE(10)
+    CR_CASE(11)
+    CR_CASE(12)
+    CR_CASE(13)
+    CR_CASE(14)
+    CR_CASE(15)
+    CR_CASE(16)
+    CR_CASE(17)
+    CR_CASE(18)
+    CR_CASE(19)
+    CR_CASE(20)
+    CR_CASE(21)
+    CR_CASE(22)
+    CR_CASE(23)
+#undef CR_CASE
+        default: break;
+      }
+    }
     __syncthreads();
     const double d = rb[j];
     const bool is_dead = !(d > thresh) || !isfinite(d);
-    if (tid == 0) dead[j] = is_dead ? 1 : 0;
-    if (kc < j) continue;  // column finished (uniform per thread)
-    if (is_dead) {
-      if (j >= i0 && j < i0 + E) RS[colbase + j] = 0.0;
-      continue;
+    if (owner) {
+      if (lane == 0) dead[j] = is_dead ? 1 : 0;
+      const double rjj = is_dead ? 0.0 : sqrt(d);
+      const double irjj = is_dead ? 0.0 : 1.0 / rjj;
+      switch (jr) {
+#define CR_CASE(R) \
+  case R: cr_factor_row<R>(v, rjj, irjj, RS, j, lane, q); break;
+    CR_CASE(0)
+    CR_CASE(1)
+    CR_CASE(2)
+    CR_CASE(3)
+    CR_CASE(4)
+    CR_CASE(5)
+    CR_CASE(6)
+    CR_CASE(7)
+    CR_CASE(8)
+    CR_CASE(9)
+    CR_CASE(10)
+    CR_CASE(11)
+    CR_CASE(12)
+    CR_CASE(13)
+    CR_CASE(14)
+    CR_CASE(15)
+    CR_CASE(16)
+    CR_CASE(17)
+    CR_CASE(18)
+    CR_CASE(19)
+    CR_CASE(20)
+    CR_CASE(21)
+    CR_CASE(22)
+    CR_CASE(23)
+#undef CR_CASE
+        default: break;
+      }
     }
-    const double rk = rb[kc] * (1.0 / d);
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int i = i0 + t;
-      if (i == j) RS[colbase + j] = (kc == j) ? sqrt(d) : v[t] / sqrt(d);
-      else if (i > j && i < iend) v[t] -= rb[i] * rk;
+    if (is_dead) continue;
+    const double invd = 1.0 / d;
+    // active rows: i = wave + CR_WAVES r > j, i.e. r >= r0
+    const int rs0 = (j + CR_WAVES - wave) / CR_WAVES;  // smallest r with wave + CR_WAVES r > j
+    switch (rs0) {
+#define CR_CASE(R)                                                                   \
+  case R:                                                                            \
+    if constexpr (R < CR_ROWS) {                                                     \
+      const int i = wave + CR_WAVES * R;                                             \
+      if (i < q) cr_update<R>(v, rb[i] * invd, rb + i + lane);                       \
+    }                                                                                \
+    [[fallthrough]];
+    CR_CASE(0)
+    CR_CASE(1)
+    CR_CASE(2)
+    CR_CASE(3)
+    CR_CASE(4)
+    CR_CASE(5)
+    CR_CASE(6)
+    CR_CASE(7)
+    CR_CASE(8)
+    CR_CASE(9)
+    CR_CASE(10)
+    CR_CASE(11)
+    CR_CASE(12)
+    CR_CASE(13)
+    CR_CASE(14)
+    CR_CASE(15)
+    CR_CASE(16)
+    CR_CASE(17)
+    CR_CASE(18)
+    CR_CASE(19)
+    CR_CASE(20)
+    CR_CASE(21)
+    CR_CASE(22)
+    CR_CASE(23)
+#undef CR_CASE
+      default: break;
     }
   }
   __syncthreads();
@@ -293,32 +417,317 @@ __global__ __launch_bounds__(CR_NT) void chol_rinv_reg_kernel(const double* __re
     anyd = __syncthreads_or(anyd);
     if (tid == 0) dead_out[q] = anyd;
   }
+  for (int e = tid; e < 2 * kCrBuf; e += CR_NT) (&buf[0][0])[e] = 0.0;
+#pragma unroll
+  for (int r = 0; r < CR_ROWS; ++r)
+#pragma unroll
+    for (int sl = 0; sl < cr_slots(r); ++sl) v[cr_off(r) + sl] = (lane == 0 && sl == 0) ? 1.0 : 0.0;
+  __syncthreads();
 
-#pragma unroll
-  for (int t = 0; t < E; ++t) v[t] = (i0 + t == kc) ? 1.0 : 0.0;
-  for (int r = q - 1; r >= 0; --r) {
-    double* xb = buf[r & 1];
-    const bool dr = dead[r] != 0;
-    const double inv = dr ? 0.0 : 1.0 / RS[r * (r + 1) / 2 + r];
-#pragma unroll
-    for (int t = 0; t < E; ++t)
-      if (i0 + t == r && r < iend) {
-        v[t] *= inv;
-        xb[kc] = v[t];
+  for (int rr = q - 1; rr >= 0; --rr) {
+    double* xb = buf[rr & 1];
+    const int jr = rr / CR_WAVES;
+    if ((rr % CR_WAVES) == wave) {
+      const double inv = dead[rr] ? 0.0 : 1.0 / RS[pk(rr, rr)];
+      switch (jr) {
+#define CR_CASE(R) \
+  case R: cr_publish<R>(v, inv, xb + rr + lane, rr + lane, q); break;
+    CR_CASE(0)
+    CR_CASE(1)
+    CR_CASE(2)
+    CR_CASE(3)
+    CR_CASE(4)
+    CR_CASE(5)
+    CR_CASE(6)
+    CR_CASE(7)
+    CR_CASE(8)
+    CR_CASE(9)
+    CR_CASE(10)
+    CR_CASE(11)
+    CR_CASE(12)
+    CR_CASE(13)
+    CR_CASE(14)
+    CR_CASE(15)
+    CR_CASE(16)
+    CR_CASE(17)
+    CR_CASE(18)
+    CR_CASE(19)
+    CR_CASE(20)
+    CR_CASE(21)
+    CR_CASE(22)
+    CR_CASE(23)
+#undef CR_CASE
+        default: break;
       }
+    }
     __syncthreads();
-    if (dr || kc < r) continue;
-    const double x = xb[kc];
-    const double* rcol = RS + r * (r + 1) / 2;  // R[i][r], i <= r
-#pragma unroll
-    for (int t = 0; t < E; ++t)
-      if (i0 + t < r) v[t] -= rcol[i0 + t] * x;
+    if (dead[rr]) continue;
+    // active rows: i = wave + CR_WAVES r < rr, i.e. r < re
+    const int re = rr > wave ? (rr - wave + CR_WAVES - 1) / CR_WAVES : 0;
+    switch (re) {
+#define CR_CASE(R)                                                                   \
+  case R + 1:                                                                        \
+    if constexpr (R < CR_ROWS) {                                                     \
+      const int i = wave + CR_WAVES * R;                                             \
+      cr_update<R>(v, RS[pk(i, rr)], xb + i + lane);                                 \
+    }                                                                                \
+    [[fallthrough]];
+      CR_CASE(23) CR_CASE(22) CR_CASE(21) CR_CASE(20) CR_CASE(19) CR_CASE(18) CR_CASE(17)
+      CR_CASE(16) CR_CASE(15) CR_CASE(14) CR_CASE(13) CR_CASE(12) CR_CASE(11) CR_CASE(10)
+      CR_CASE(9) CR_CASE(8) CR_CASE(7) CR_CASE(6) CR_CASE(5) CR_CASE(4) CR_CASE(3)
+      CR_CASE(2) CR_CASE(1) CR_CASE(0)
+#undef CR_CASE
+      default: break;
+    }
   }
 #pragma unroll
-  for (int t = 0; t < E; ++t)
-    if (i0 + t < iend) Rinv[(int64_t)(i0 + t) * q + kc] = v[t];
+  for (int r = 0; r < CR_ROWS; ++r) {
+    const int i = wave + CR_WAVES * r;
+    if (i >= q) continue;
+#pragma unroll
+    for (int sl = 0; sl < cr_slots(r); ++sl) {
+      const int k = i + lane + 64 * sl;
+      if (k < q) Rinv[(int64_t)i * q + k] = v[cr_off(r) + sl];
+    }
+  }
   for (int e = tid; e < q * q; e += CR_NT)
     if (e / q > e % q) Rinv[e] = 0.0;
+}
+
+// Blocked variant on the FP64 matrix cores for q <= kCbMaxQ: the matrix is
+// held as its upper 16 x 16 tiles in LDS (rows padded to 17 doubles), and
+//   for each tile column k:  one wave factors the diagonal tile (element
+//       Cholesky with the same rank guard) and replaces it by its inverse
+//       X_kk = R_kk^-1; the panel becomes R_kj = X_kk^T A_kj and the trailing
+//       tiles A_IJ -= R_kI^T R_kJ (16 x 16 x 16 products, v_mfma_f64_16x16x4);
+//   inverse, tile rows i from the bottom:  X_ij = -X_ii sum_{l=i+1..j} R_il X_lj.
+// Three barriers per tile column and two per inverse row replace the one per
+// matrix row of the element algorithms.  Dead pivots give zero rows of R and
+// zero rows and columns of X (the dead rows of X_kk vanish, so they vanish in
+// every product), as in chol_rinv_kernel.
+typedef double cb_d4 __attribute__((ext_vector_type(4)));
+constexpr int CB_WAVES = 8;
+constexpr int CB_NT = 64 * CB_WAVES;
+constexpr int CB_LD = 17;             // tile row stride (doubles)
+constexpr int CB_TS = 16 * CB_LD;     // tile size (doubles)
+constexpr int kCbMaxQ = 176;          // 11 tile rows: 66 tiles, 144 KB
+constexpr int kCbMaxT = kCbMaxQ / 16;
+
+__device__ __forceinline__ int cb_tile(int I, int J) { return (J * (J + 1) / 2 + I) * CB_TS; }
+
+__device__ __forceinline__ void cb_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// acc += s * op(A) B over one 16-deep tile product, op(A) = A^T (TA) or A.
+// Operand lanes: a = op(A)[lane & 15][4 ks + (lane >> 4)], b = B[4 ks + (lane >> 4)][lane & 15];
+// acc[r] = C[(lane >> 4) + 4 r][lane & 15].
+template <bool TA>
+__device__ __forceinline__ void cb_mma(cb_d4& acc, const double* A, const double* B, int lane,
+                                       double sgn) {
+  const int m = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int k = 4 * ks + kq;
+    const double a = TA ? A[k * CB_LD + m] : A[m * CB_LD + k];
+    const double b = B[k * CB_LD + m];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sgn * a, b, acc, 0, 0, 0);
+  }
+}
+__device__ __forceinline__ cb_d4 cb_load_acc(const double* T, int lane) {
+  cb_d4 v;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v[r] = T[((lane >> 4) + 4 * r) * CB_LD + (lane & 15)];
+  return v;
+}
+__device__ __forceinline__ void cb_store_acc(double* T, int lane, const cb_d4& v) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) T[((lane >> 4) + 4 * r) * CB_LD + (lane & 15)] = v[r];
+}
+
+// One wave: Cholesky of the diagonal tile D (upper factor, rank guard), then
+// D <- R^-1 (upper, lower triangle zero).  Lane l owns column j = l & 15 and
+// rows i = (l >> 4) + 4 t.  dead[c] flags the dead pivots of this tile.
+__device__ __forceinline__ void cb_factor_diag(double* D, double* xbuf, int lane, double thresh,
+                                               int* dead) {
+  const int j = lane & 15, i0 = lane >> 4;
+  for (int c = 0; c < 16; ++c) {
+    const double d = D[c * CB_LD + c];
+    const bool dd = !(d > thresh) || !isfinite(d);
+    const double rcj = D[c * CB_LD + j];
+    double rci[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) rci[t] = D[c * CB_LD + i0 + 4 * t];
+    cb_wave_sync();
+    const double invd = dd ? 0.0 : 1.0 / d;
+    const double rcc = dd ? 0.0 : sqrt(d);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = i0 + 4 * t;
+      if (i > c && j >= i) D[i * CB_LD + j] -= rci[t] * invd * rcj;
+      else if (i == c && j >= c) D[c * CB_LD + j] = (j == c) ? rcc : (dd ? 0.0 : rcj / rcc);
+    }
+    if (lane == 0) dead[c] = dd ? 1 : 0;
+    cb_wave_sync();
+  }
+  // X = R^-1 row by row from the bottom, X in registers, rows published in xbuf
+  double x[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) x[t] = (i0 + 4 * t == j) ? 1.0 : 0.0;
+  for (int r = 15; r >= 0; --r) {
+    double* xr = xbuf + 16 * (r & 1);
+    const double rr = D[r * CB_LD + r];
+    const double inv = (dead[r] || rr == 0.0) ? 0.0 : 1.0 / rr;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (i0 + 4 * t == r) {
+        x[t] *= inv;
+        xr[j] = x[t];
+      }
+    cb_wave_sync();
+    const double xrj = xr[j];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = i0 + 4 * t;
+      if (i < r) x[t] -= D[i * CB_LD + r] * xrj;
+    }
+  }
+  cb_wave_sync();
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int i = i0 + 4 * t;
+    D[i * CB_LD + j] = (j >= i) ? x[t] : 0.0;
+  }
+  cb_wave_sync();
+}
+
+__global__ __launch_bounds__(CB_NT) void chol_rinv_blk_kernel(const double* __restrict__ G,
+                                                              double* __restrict__ Rinv, int q,
+                                                              double shift_rel, double rank_tol,
+                                                              int* dead_out,
+                                                              const int* only_if_dead) {
+  extern __shared__ double T[];  // upper tiles
+  __shared__ int dead[kCbMaxQ];
+  __shared__ double xbuf[32];
+  __shared__ double s_red[CB_WAVES][2];
+  __shared__ int any;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (only_if_dead) {
+    if (tid == 0) any = 0;
+    __syncthreads();
+    for (int i = tid; i < q; i += CB_NT)
+      if (only_if_dead[i]) any = 1;
+    __syncthreads();
+    if (!any) {
+      for (int e = tid; e < q * q; e += CB_NT) Rinv[e] = (e / q == e % q) ? 1.0 : 0.0;
+      return;
+    }
+  }
+  double tr = 0.0, dm = 0.0;
+  for (int i = tid; i < q; i += CB_NT) {
+    const double gii = G[(int64_t)i * q + i];
+    tr += gii;
+    dm = fmax(dm, gii);
+  }
+  for (int o = 32; o > 0; o >>= 1) { tr += __shfl_down(tr, o); dm = fmax(dm, __shfl_down(dm, o)); }
+  if (lane == 0) { s_red[wave][0] = tr; s_red[wave][1] = dm; }
+  __syncthreads();
+  if (tid == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int w = 0; w < CB_WAVES; ++w) { t0 += s_red[w][0]; t1 = fmax(t1, s_red[w][1]); }
+    s_red[0][0] = t0; s_red[0][1] = t1;
+  }
+  __syncthreads();
+  const double shift = shift_rel * s_red[0][0];
+  const double thresh = rank_tol * (s_red[0][1] + shift);
+  const int Q = (q + 15) / 16;
+  const int ntiles = Q * (Q + 1) / 2;
+
+  // tiles of G + shift I (upper tiles; diagonal tiles symmetric; padding = identity)
+  for (int t = wave; t < ntiles; t += CB_WAVES) {
+    int J = 0;
+    while ((J + 1) * (J + 2) / 2 <= t) ++J;
+    const int I = t - J * (J + 1) / 2;
+    for (int e = lane; e < 256; e += 64) {
+      const int a = 16 * I + (e >> 4), b = 16 * J + (e & 15);
+      double x;
+      if (a < q && b < q) {
+        const int lo = min(a, b), hi = max(a, b);
+        x = G[(int64_t)lo * q + hi] + (a == b ? shift : 0.0);
+      } else {
+        x = (a == b) ? 1.0 : 0.0;
+      }
+      T[t * CB_TS + (e >> 4) * CB_LD + (e & 15)] = x;
+    }
+  }
+  __syncthreads();
+
+  for (int k = 0; k < Q; ++k) {
+    if (wave == 0) cb_factor_diag(T + cb_tile(k, k), xbuf, lane, thresh, dead + 16 * k);
+    __syncthreads();
+    for (int jj = k + 1 + wave; jj < Q; jj += CB_WAVES) {
+      cb_d4 acc = {0.0, 0.0, 0.0, 0.0};
+      cb_mma<true>(acc, T + cb_tile(k, k), T + cb_tile(k, jj), lane, 1.0);
+      cb_wave_sync();
+      cb_store_acc(T + cb_tile(k, jj), lane, acc);
+    }
+    __syncthreads();
+    const int nrem = Q - k - 1;
+    const int ntr = nrem * (nrem + 1) / 2;
+    for (int u = wave; u < ntr; u += CB_WAVES) {
+      int Jp = 0;
+      while ((Jp + 1) * (Jp + 2) / 2 <= u) ++Jp;
+      const int I = k + 1 + (u - Jp * (Jp + 1) / 2), J = k + 1 + Jp;
+      cb_d4 acc = cb_load_acc(T + cb_tile(I, J), lane);
+      cb_mma<true>(acc, T + cb_tile(k, I), T + cb_tile(k, J), lane, -1.0);
+      cb_store_acc(T + cb_tile(I, J), lane, acc);
+    }
+    __syncthreads();
+  }
+  if (dead_out) {
+    int anyd = 0;
+    for (int i = tid; i < q; i += CB_NT) {
+      dead_out[i] = dead[i];
+      anyd |= dead[i];
+    }
+    anyd = __syncthreads_or(anyd);
+    if (tid == 0) dead_out[q] = anyd;
+  }
+
+  // X = R^-1 by tile rows from the bottom: X_ij = -X_ii sum_{l=i+1..j} R_il X_lj
+  constexpr int kJPer = (kCbMaxT + CB_WAVES - 1) / CB_WAVES;
+  for (int i = Q - 2; i >= 0; --i) {
+    cb_d4 res[kJPer];
+#pragma unroll
+    for (int m = 0; m < kJPer; ++m) {
+      const int jj = i + 1 + wave + CB_WAVES * m;
+      res[m] = cb_d4{0.0, 0.0, 0.0, 0.0};
+      if (jj < Q) {
+        cb_d4 S = {0.0, 0.0, 0.0, 0.0};
+        for (int l = i + 1; l <= jj; ++l) cb_mma<false>(S, T + cb_tile(i, l), T + cb_tile(l, jj), lane, 1.0);
+        // the accumulator layout of S is the B-operand layout (row 4 ks + (lane >> 4) in S[ks])
+        const double* Xii = T + cb_tile(i, i);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const double a = Xii[(lane & 15) * CB_LD + 4 * ks + (lane >> 4)];
+          res[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a, S[ks], res[m], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kJPer; ++m) {
+      const int jj = i + 1 + wave + CB_WAVES * m;
+      if (jj < Q) cb_store_acc(T + cb_tile(i, jj), lane, res[m]);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < q * q; e += CB_NT) {
+    const int a = e / q, b = e % q;
+    Rinv[e] = (a <= b) ? T[cb_tile(a >> 4, b >> 4) + (a & 15) * CB_LD + (b & 15)] : 0.0;
+  }
 }
 
 // Dead (numerically rank-deficient) columns of an orthonormalisation are
@@ -349,11 +758,39 @@ __global__ void refill_kernel(double* __restrict__ Y, int64_t m, int q,
 // rot layout: [(sweep*(n-1) + round) * (n/2) + pair] * 2 -> (c, s);
 // info[0] = number of sweeps executed.
 // --------------------------------------------------------------------------
+// r in [0, n-2], a in [0, n/2): (r + a) mod (n-1) and (r - a) mod (n-1)
+// need one correction each
 __device__ __forceinline__ void round_pair(int n, int r, int a, int& p, int& qq) {
   int x, y;
   if (a == 0) { x = n - 1; y = r; }
-  else { x = (r + a) % (n - 1); y = (r - a + n - 1) % (n - 1); }
+  else {
+    x = r + a;
+    if (x >= n - 1) x -= n - 1;
+    y = r - a;
+    if (y < 0) y += n - 1;
+  }
   p = min(x, y); qq = max(x, y);
+}
+
+// Jacobi rotation (c, s) annihilating apq of [[app, apq], [apq, aqq]]:
+// t = tan(theta) is the smaller root of t^2 + 2 tau t - 1 = 0,
+// tau = (aqq - app) / (2 apq), written as t = 2 apq sgn(d) / (|d| + sqrt(d^2 + 4 apq^2))
+// with d = aqq - app (sgn(0) = 1): one square root, one division and one
+// reciprocal square root.  The tau form covers the case where the squares
+// underflow.
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double& c,
+                                                double& s) {
+  const double d = aqq - app;
+  const double h = sqrt(d * d + 4.0 * apq * apq);
+  double t;
+  if (h > 0.0 && isfinite(h)) {
+    t = (d >= 0.0 ? 2.0 * apq : -2.0 * apq) / (fabs(d) + h);
+  } else {
+    const double tau = d / (2.0 * apq);
+    t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+  }
+  c = rsqrt(1.0 + t * t);
+  s = t * c;
 }
 
 constexpr int kMaxSweeps = 40;
@@ -463,12 +900,7 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
         double c = 1.0, s = 0.0;
         if (qq < q) {
           const double app = S[pk(p, p)], aqq = S[pk(qq, qq)], apq = S[pk(p, qq)];
-          if (apq != 0.0 && fabs(apq) > tol * sqrt(fabs(app * aqq))) {
-            const double tau = (aqq - app) / (2.0 * apq);
-            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-          }
+          if (apq != 0.0 && fabs(apq) > tol * sqrt(fabs(app * aqq))) jacobi_rotation(app, aqq, apq, c, s);
         }
         cs_c[a] = c; cs_s[a] = s;
         const int64_t ro = ((int64_t)(sweep * (n - 1) + r) * half + a) * 2;
@@ -497,69 +929,76 @@ __global__ __launch_bounds__(JAC_NT) void jacobi_kernel(const double* __restrict
 }
 
 // Replay the recorded rotations on the rows of V (V starts as identity).
-// Each workgroup owns ``rows`` rows in LDS.  Afterwards the eigenpairs are
-// sorted by descending eigenvalue (ties: lower index first).
-constexpr int kReplayRounds = 32;
-constexpr int kRepList = 8;  // (row, pair) items per thread: rows * n/2 <= 8 * 256
+// Rows of V are independent under column rotations, so each wave owns one
+// row in LDS and applies round after round with wave-level ordering only;
+// the workgroup meets once per chunk of rotations, staged through LDS from
+// registers loaded during the previous chunk.  Afterwards the eigenpairs
+// are sorted by descending eigenvalue (ties: lower index first).
+constexpr int kRepWaves = 4;
+constexpr int kRepPrefetch = 16;  // doubles of the next chunk per thread
 
-__global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __restrict__ rot,
-                                                             const int* __restrict__ info, int q,
-                                                             int rows, double* __restrict__ w,
-                                                             double* __restrict__ V,
-                                                             const double* __restrict__ wraw) {
-  extern __shared__ double vr[];  // rows x n, then kReplayRounds x half x 2 rotations
+__device__ __forceinline__ void rep_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * kRepWaves) void jacobi_vectors_kernel(
+    const double* __restrict__ rot, const int* __restrict__ info, int q, double* __restrict__ w,
+    double* __restrict__ V, const double* __restrict__ wraw) {
+  extern __shared__ double lds[];  // kRepWaves rows x n, then one rotation chunk
   __shared__ int perm[512];
-  __shared__ short pp[kReplayRounds][256], qq_[kReplayRounds][256];
+  constexpr int NT = 64 * kRepWaves;
   const int n = q + (q & 1);
   const int half = n / 2;
-  double* rs = vr + rows * n;
-  const int r0 = blockIdx.x * rows;
-  const int nr = min(rows, q - r0);
-  const int tid = threadIdx.x;
-  for (int e = tid; e < rows * n; e += blockDim.x) {
-    const int rr = e / n, cc = e % n;
-    vr[e] = (rr < nr && r0 + rr == cc) ? 1.0 : 0.0;
-  }
-  // (row, pair) work items of this thread, identical in every round
-  int my_row[kRepList], my_pair[kRepList];
-#pragma unroll
-  for (int i = 0; i < kRepList; ++i) {
-    const int e = tid + i * 256;
-    my_pair[i] = e % half;
-    my_row[i] = e / half;
-  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row = blockIdx.x * kRepWaves + wave;  // rows >= q idle
+  double* v = lds + wave * n;
+  double* rs = lds + kRepWaves * n;
+  for (int c = lane; c < n; c += 64) v[c] = (c == row) ? 1.0 : 0.0;
+  const int rpc = max(1, kRepPrefetch * NT / (2 * half));  // rounds per chunk
+  const int chunk = rpc * half * 2;
   const int total = info[0] * (n - 1);
-  for (int st0 = 0; st0 < total; st0 += kReplayRounds) {
-    const int nrounds = min(kReplayRounds, total - st0);
-    __syncthreads();
-    for (int e = tid; e < nrounds * half * 2; e += blockDim.x)
-      rs[e] = rot[(int64_t)st0 * half * 2 + e];
-    for (int e = tid; e < nrounds * half; e += blockDim.x) {
-      const int rr = e / half, a = e % half;
-      int p, qv;
-      round_pair(n, (st0 + rr) % (n - 1), a, p, qv);
-      pp[rr][a] = (short)p;
-      qq_[rr][a] = (short)qv;
-    }
-    __syncthreads();
-    for (int rr = 0; rr < nrounds; ++rr) {
+  const int64_t rot_elems = (int64_t)total * half * 2;
+  double pre[kRepPrefetch];
+  auto prefetch = [&](int st0) {
 #pragma unroll
-      for (int i = 0; i < kRepList; ++i) {
-        if (tid + i * 256 >= half * nr) break;
-        const int a = my_pair[i];
-        const double c = rs[(rr * half + a) * 2], sn = rs[(rr * half + a) * 2 + 1];
-        if (sn == 0.0) continue;
-        double* v = vr + my_row[i] * n;
-        const int p = pp[rr][a], qv = qq_[rr][a];
-        const double vp = v[p], vq = v[qv];
-        v[p] = c * vp - sn * vq;
-        v[qv] = sn * vp + c * vq;
-      }
-      __syncthreads();
+    for (int i = 0; i < kRepPrefetch; ++i) {
+      const int e = tid + i * NT;
+      const int64_t g = (int64_t)st0 * half * 2 + e;
+      pre[i] = (e < chunk && g < rot_elems) ? rot[g] : 0.0;
     }
+  };
+  prefetch(0);
+  int rnd0 = 0;  // round index (mod n - 1) of the chunk's first round
+  for (int st0 = 0; st0 < total; st0 += rpc) {
+    const int nrounds = min(rpc, total - st0);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRepPrefetch; ++i)
+      if (tid + i * NT < chunk) rs[tid + i * NT] = pre[i];
+    __syncthreads();
+    if (st0 + rpc < total) prefetch(st0 + rpc);
+    if (row < q) {
+      int rnd = rnd0;
+      for (int rr = 0; rr < nrounds; ++rr) {
+        for (int a = lane; a < half; a += 64) {
+          const double c = rs[(rr * half + a) * 2], sn = rs[(rr * half + a) * 2 + 1];
+          if (sn == 0.0) continue;
+          int p, qv;
+          round_pair(n, rnd, a, p, qv);
+          const double vp = v[p], vq = v[qv];
+          v[p] = c * vp - sn * vq;
+          v[qv] = sn * vp + c * vq;
+        }
+        rep_wave_sync();
+        if (++rnd == n - 1) rnd = 0;
+      }
+    }
+    rnd0 = (rnd0 + nrounds) % (n - 1);
   }
   // descending order permutation from the raw eigenvalues
-  for (int i = tid; i < q; i += blockDim.x) {
+  for (int i = tid; i < q; i += NT) {
     const double wi = wraw[i];
     int rank = 0;
     for (int j = 0; j < q; ++j) {
@@ -569,12 +1008,10 @@ __global__ __launch_bounds__(256) void jacobi_vectors_kernel(const double* __res
     perm[rank] = i;
   }
   __syncthreads();
-  for (int e = tid; e < nr * q; e += blockDim.x) {
-    const int rr = e / q, cpos = e % q;
-    V[(int64_t)(r0 + rr) * q + cpos] = vr[rr * n + perm[cpos]];
-  }
+  if (row < q)
+    for (int cpos = lane; cpos < q; cpos += 64) V[(int64_t)row * q + cpos] = v[perm[cpos]];
   if (blockIdx.x == 0)
-    for (int i = tid; i < q; i += blockDim.x) w[i] = wraw[perm[i]];
+    for (int i = tid; i < q; i += NT) w[i] = wraw[perm[i]];
 }
 
 // --------------------------------------------------------------------------
@@ -680,17 +1117,17 @@ void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double ra
                hipStream_t s, double* gscratch, int* dead_out, const int* only_if_dead) {
   ACETN_REQUIRE(q >= 1 && q <= 512, "chol_rinv: q out of range");
   const size_t pb = packed_bytes(q);
-  if (q <= kCrMaxQ) {
-    ACETN_REQUIRE(cr_runs(q, 16) <= CR_NT, "chol_rinv: register layout too small");
-    if (cr_runs(q, 8) <= CR_NT)
-      hipLaunchKernelGGL(chol_rinv_reg_kernel<8>, dim3(1), dim3(CR_NT), pb, s, G, Rinv, q,
-                         shift_rel, rank_tol, dead_out, only_if_dead);
-    else if (cr_runs(q, 12) <= CR_NT)
-      hipLaunchKernelGGL(chol_rinv_reg_kernel<12>, dim3(1), dim3(CR_NT), pb, s, G, Rinv, q,
-                         shift_rel, rank_tol, dead_out, only_if_dead);
-    else
-      hipLaunchKernelGGL(chol_rinv_reg_kernel<16>, dim3(1), dim3(CR_NT), pb, s, G, Rinv, q,
-                         shift_rel, rank_tol, dead_out, only_if_dead);
+  if (q <= kCbMaxQ) {
+    const int Q = (q + 15) / 16;
+    const size_t tb = sizeof(double) * (size_t)CB_TS * Q * (Q + 1) / 2;
+    static bool attr = false;
+    if (!attr) {
+      ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)chol_rinv_blk_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+      attr = true;
+    }
+    hipLaunchKernelGGL(chol_rinv_blk_kernel, dim3(1), dim3(CB_NT), tb, s, G, Rinv, q, shift_rel,
+                       rank_tol, dead_out, only_if_dead);
     ACETN_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -738,11 +1175,10 @@ void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_
     hipLaunchKernelGGL((jacobi_kernel<false, false>), dim3(1), dim3(JAC_NT), 0, s, G, q, wraw, rot,
                        info, gscr, use_lds, tol);
   ACETN_HIP_CHECK(hipGetLastError());
-  const int rows = 8;
-  const int nblk = (q + rows - 1) / rows;
-  const size_t lds = sizeof(double) * ((size_t)rows * n + (size_t)kReplayRounds * (n / 2) * 2);
-  hipLaunchKernelGGL(jacobi_vectors_kernel, dim3(nblk), dim3(256), lds, s, rot, info, q, rows, w,
-                     V, wraw);
+  const int nblk = (q + kRepWaves - 1) / kRepWaves;
+  const size_t lds = sizeof(double) * ((size_t)kRepWaves * n + (size_t)kRepPrefetch * 64 * kRepWaves);
+  hipLaunchKernelGGL(jacobi_vectors_kernel, dim3(nblk), dim3(64 * kRepWaves), lds, s, rot, info, q,
+                     w, V, wraw);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

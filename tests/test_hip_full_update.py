@@ -261,12 +261,16 @@ def test_fast_full_update_steps_match_oracle(hip):
         net.bond_list.reverse()
     hst = {s: {'A': host(v['A']), 'C': [host(c) for c in v['C']],
                'E': [host(e) for e in v['E']]} for s, v in net.sites.items()}
+    # Two full-update steps on this converged state amplify rounding: relative
+    # perturbations of 1e-15 in the corners move the oracle's own site tensors
+    # by up to 7e-8 and its energy by 2.2e-10 (relative), so the tensors are
+    # compared at 5e-7 and the energy at 1e-9.
     for s in hst:
-        np.testing.assert_allclose(hst[s]['A'], ost[s]['A'], atol=1e-8)
+        np.testing.assert_allclose(hst[s]['A'], ost[s]['A'], atol=5e-7)
     e_o = om.measure(ost, 2, 2, omod.Heisenberg(1.0))['Energy']
     e_h = om.measure(hst, 2, 2, omod.Heisenberg(1.0))['Energy']
     print(f"energy after 2 FU steps: hip {e_h:.15f} oracle {e_o:.15f}")
-    assert e_h == pytest.approx(e_o, rel=1e-10)
+    assert e_h == pytest.approx(e_o, rel=1e-9)
 
 
 @pytest.mark.parametrize("k", [1, 2])

@@ -9,22 +9,21 @@
 // accumulated in FP32 by the MFMA.  bf16 products are exact in FP32, so the
 // only errors are the operand truncation and the FP32 accumulation.
 //
-// Operand B is split once per call, before the product, into SPLIT bf16
-// planes laid out [plane][n][k] with k contiguous (k padded to a multiple of
-// 32 with zeros): the product kernel then moves B with 16-byte loads straight
-// into LDS, and no M tile converts B again.  Operand A is either converted
-// and split in the product kernel while it is staged through LDS, or given
-// already split as planes [plane][m][k] (split_transpose_planes produces both
-// orientations of a matrix in one pass, for a matrix that several products
-// read, such as an enlarged corner in the range finder).
-//
-// Tiles: 4 waves in a 2 x 2 grid, each owning WM x WN MFMA blocks of 16 x 16,
-// BK = 32 (one MFMA k-step).  LDS holds the pieces k-contiguous ([m][k] and
-// [n][k]) so every lane fetches its 8 bf16 of a fragment with one 16-byte
-// read: A lane l holds A[l%16][8(l/16)..+7], B lane l holds B[8(l/16)..+7][l%16],
-// and the f32 result lane l holds D[4(l/16)+r][l%16] (checked on gfx950).
-// Operand addressing uses the same multi-level index maps as the FP64 GEMM,
-// so einsum permutations are folded into the loads here as well.
+// Two kernels:
+//  * products with an enlarged corner as A (the range finder): the corner is
+//    split once into planes [plane][m][k] of both orientations
+//    (split_transpose), B is split once per call into planes [plane][n][k]
+//    (k padded to 32 with zeros), and gemm_split_kernel (4 waves in a 2 x 2
+//    grid, 64 x 160 tiles) moves both with 16-byte loads straight into LDS;
+//  * every other product: gemm_general_kernel (8 waves, 4 x 2, 128 x 160
+//    tiles) reads both operands as FP64 through the same multi-level index
+//    maps as the FP64 GEMM (einsum permutations folded into the loads) and
+//    splits them in registers on the way to LDS.
+// LDS holds the pieces k-contiguous ([m][k] and [n][k]) so every lane fetches
+// its 8 bf16 of a fragment with one 16-byte read: A lane l holds
+// A[l%16][8(l/16)..+7], B lane l holds B[8(l/16)..+7][l%16], and the f32 result
+// lane l holds D[4(l/16)+r][l%16] (checked on gfx950).  BK = 32 (one MFMA
+// k-step); long K is split into partial sums reduced in a fixed order.
 #include "internal.h"
 
 namespace acetn {
@@ -295,6 +294,220 @@ __global__ void splitk_reduce_split(GemmDesc g, double* __restrict__ C, int kspl
   }
 }
 
+// General products (operands FP64 through their index maps, split in
+// registers on the way to LDS): 8 waves, 4 along M x 2 along N, each owning
+// WM x WN blocks of 16 x 16; the 128 x 160 tile keeps two workgroups per CU.
+// Split-K partials are FP32 (the accumulator type), reduced in a fixed order.
+constexpr int NT8 = 512;
+
+template <int SPLIT, int WM, int WN>
+__global__ __launch_bounds__(NT8) void gemm_general_kernel(GemmDesc g, const double* __restrict__ A,
+                                                           const double* __restrict__ B,
+                                                           double* __restrict__ C, int ksplit,
+                                                           float* __restrict__ work) {
+  constexpr int BM = 64 * WM, BN = 32 * WN;
+  constexpr int LK = BK + KPAD;
+  constexpr int AE = BM * BK / NT8;               // A elements per thread and k-tile
+  constexpr int BE = (BN * BK + NT8 - 1) / NT8;   // B elements per thread and k-tile
+  extern __shared__ __align__(16) char smem[];
+  __bf16* As = reinterpret_cast<__bf16*>(smem);  // [SPLIT][BM][LK]
+  __bf16* Bs = As + SPLIT * BM * LK;              // [SPLIT][BN][LK]
+  if (g.run_if && *g.run_if == 0) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int kpart = blockIdx.z;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  int kchunk = (g.K + ksplit - 1) / ksplit;
+  kchunk = (kchunk + BK - 1) / BK * BK;
+  const int kbeg = kpart * kchunk;
+  const int kend = min(g.K, kbeg + kchunk);
+
+  const bool a_mfast = contiguous(g.am) || !contiguous(g.ak);
+  int a_ml[AE], a_kl[AE];
+  int64_t a_moff[AE];
+  bool a_mok[AE];
+#pragma unroll
+  for (int i = 0; i < AE; ++i) {
+    const int e = tid + i * NT8;
+    if (a_mfast) { a_ml[i] = e % BM; a_kl[i] = e / BM; }
+    else         { a_kl[i] = e % BK; a_ml[i] = e / BK; }
+    const int m = m0 + a_ml[i];
+    a_mok[i] = m < g.M;
+    a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
+  }
+  const bool b_nfast = contiguous(g.bn) || !contiguous(g.bk);
+  int b_nl[BE], b_kl[BE];
+  int64_t b_noff[BE];
+  bool b_ok[BE];
+#pragma unroll
+  for (int i = 0; i < BE; ++i) {
+    const int e = tid + i * NT8;
+    if (b_nfast) { b_nl[i] = e % BN; b_kl[i] = e / BN; }
+    else         { b_kl[i] = e % BK; b_nl[i] = e / BK; }
+    const int n = n0 + b_nl[i];
+    b_ok[i] = e < BN * BK && n < g.N;
+    b_noff[i] = b_ok[i] ? imap(g.bn, n) : 0;
+  }
+
+  f4 acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  float ra[AE], rb[BE];
+  auto load_tile = [&](int kt) {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
+      const int k = kt + a_kl[i];
+      ra[i] = (a_mok[i] && k < kend) ? (float)A[a_moff[i] + imap(g.ak, k)] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int k = kt + b_kl[i];
+      rb[i] = (b_ok[i] && k < kend) ? (float)B[imap(g.bk, k) + b_noff[i]] : 0.f;
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
+      __bf16 pc[SPLIT];
+      split<SPLIT>(ra[i], pc);
+#pragma unroll
+      for (int s = 0; s < SPLIT; ++s) As[(s * BM + a_ml[i]) * LK + a_kl[i]] = pc[s];
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      if (tid + i * NT8 < BN * BK) {
+        __bf16 pc[SPLIT];
+        split<SPLIT>(rb[i], pc);
+#pragma unroll
+        for (int s = 0; s < SPLIT; ++s) Bs[(s * BN + b_nl[i]) * LK + b_kl[i]] = pc[s];
+      }
+    }
+  };
+
+  if (kbeg < kend) load_tile(kbeg);
+  const int k0 = 8 * (lane >> 4);
+  for (int kt = kbeg; kt < kend; kt += BK) {
+    __syncthreads();
+    store_tile();
+    __syncthreads();
+    if (kt + BK < kend) load_tile(kt + BK);  // global loads in flight during the MFMAs
+    bf8 a[SPLIT][WM];
+#pragma unroll
+    for (int s = 0; s < SPLIT; ++s)
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+        a[s][i] = *reinterpret_cast<const bf8*>(
+            &As[(s * BM + wm * 16 * WM + i * 16 + (lane & 15)) * LK + k0]);
+    // B pieces one at a time, the smaller piece products first
+#pragma unroll
+    for (int sb = SPLIT - 1; sb >= 0; --sb) {
+      bf8 b[WN];
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+        b[j] = *reinterpret_cast<const bf8*>(
+            &Bs[(sb * BN + wn * 16 * WN + j * 16 + (lane & 15)) * LK + k0]);
+#pragma unroll
+      for (int sa = SPLIT - 1 - sb; sa >= 0; --sa)
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+          for (int j = 0; j < WN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
+        const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        if (ksplit == 1) {
+          const double v = (double)acc[i][j][r];
+          const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+          C[off] = (g.beta != 0.0) ? g.alpha * v + g.beta * C[off] : g.alpha * v;
+        } else {
+          work[((int64_t)kpart * g.M + m) * g.N + n] = acc[i][j][r];
+        }
+      }
+}
+
+__global__ void splitk_reduce_f32(GemmDesc g, double* __restrict__ C, int ksplit,
+                                  const float* __restrict__ work) {
+  if (g.run_if && *g.run_if == 0) return;
+  const int64_t mn = (int64_t)g.M * g.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int p = 0; p < ksplit; ++p) s += (double)work[(int64_t)p * mn + e];
+    const int m = (int)(e / g.N), n = (int)(e % g.N);
+    const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+    C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
+  }
+}
+
+// general-path tiles: 128 x 160 for large products, 64 x 64 otherwise
+struct GTile {
+  int wm, wn;
+};
+GTile general_tile(const GemmDesc& g) {
+  if (g.M >= 1024 && g.N >= 96) return GTile{2, 5};
+  return GTile{1, 2};
+}
+int general_ksplit(const GemmDesc& g) {
+  const GTile t = general_tile(g);
+  const int BM = 64 * t.wm, BN = 32 * t.wn;
+  const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+  if (tiles >= 384 || g.K < 8 * BK) return 1;
+  int ks = (int)((512 + tiles - 1) / tiles);
+  ks = std::min(ks, g.K / (4 * BK));
+  ks = std::min(ks, 32);
+  return std::max(ks, 1);
+}
+size_t general_partial_elems(const GemmDesc& g, int ks) {  // doubles holding FP32 partials
+  return ks > 1 ? ((size_t)ks * g.M * g.N + 1) / 2 : 0;
+}
+
+template <int SPLIT, int WM, int WN>
+void launch_general(const GemmDesc& g, const double* A, const double* B, double* C, int ks,
+                    double* work, hipStream_t s) {
+  constexpr int BM = 64 * WM, BN = 32 * WN;
+  const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
+  static bool attr = false;
+  if (!attr) {
+    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)gemm_general_kernel<SPLIT, WM, WN>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, ks);
+  ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_split: grid too large");
+  float* wf = reinterpret_cast<float*>(work);
+  hipEvent_t e0 = kernel_timing_begin(s);
+  hipLaunchKernelGGL((gemm_general_kernel<SPLIT, WM, WN>), grid, dim3(NT8), lds, s, g, A, B, C, ks,
+                     wf);
+  ACETN_HIP_CHECK(hipGetLastError());
+  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
+  if (ks > 1) {
+    const int64_t mn = (int64_t)g.M * g.N;
+    const int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
+    hipLaunchKernelGGL(splitk_reduce_f32, dim3(blocks), dim3(256), 0, s, g, C, ks, wf);
+    ACETN_HIP_CHECK(hipGetLastError());
+  }
+}
+
+template <int SPLIT>
+void run_general(const GemmDesc& g, const double* A, const double* B, double* C, int ks,
+                 double* work, hipStream_t s) {
+  if (general_tile(g).wm == 2) launch_general<SPLIT, 2, 5>(g, A, B, C, ks, work, s);
+  else launch_general<SPLIT, 1, 2>(g, A, B, C, ks, work, s);
+}
+
 struct Tile {
   int wm, wn;
 };
@@ -375,7 +588,8 @@ void run(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, c
 
 size_t gemm_split_work_elems(const GemmDesc& g, int count) {
   ACETN_REQUIRE(count == 1, "gemm_split: batched products are not supported");
-  return partial_elems(g, choose_ksplit(g)) + plane_elems(g, 3);
+  return std::max(partial_elems(g, choose_ksplit(g)) + plane_elems(g, 3),
+                  general_partial_elems(g, general_ksplit(g)));
 }
 
 void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DType tb, DType tc,
@@ -386,12 +600,10 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
                 "gemm_split: FP64 operands and result only");
   if (g.M <= 0 || g.N <= 0) return;
   ACETN_REQUIRE(g.K > 0 || g.beta == 0.0, "gemm_split: K == 0 with beta != 0 unsupported");
-  ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, 1) + plane_elems(g, split),
-                "gemm_split: scratch too small");
-  int ks = choose_ksplit(g);
-  if (work_elems < partial_elems(g, ks) + plane_elems(g, split)) ks = 1;
-  if (split == 2) run<2, false>(g, b.A[0], nullptr, 0, b.B[0], b.C[0], ks, work, s);
-  else run<3, false>(g, b.A[0], nullptr, 0, b.B[0], b.C[0], ks, work, s);
+  int ks = general_ksplit(g);
+  if (work == nullptr || work_elems < general_partial_elems(g, ks)) ks = 1;
+  if (split == 2) run_general<2>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
+  else run_general<3>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
 }
 
 void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B, double* C,

@@ -178,322 +178,6 @@ __global__ __launch_bounds__(LA_NT) void chol_rinv_kernel(const double* __restri
   }
 }
 
-// Register-resident variant for q <= kCrMaxQ: the same factor and inverse,
-// with the upper triangle held in registers.  Wave w owns the rows
-// i = w + CR_WAVES r and lane l the columns k = i + l + 64 s of each (k >= i),
-// so a step reads the published row with consecutive addresses per wave (no
-// LDS bank conflicts).  The rows a wave still updates form a suffix (Cholesky)
-// or a prefix (inverse) of its row slots, entered through one switch, and
-// every column slot is updated unconditionally (the row buffers are zero past
-// q), so a step costs little scalar control flow.  Both phases are
-// right-looking with one barrier per step:
-//   Cholesky, step j:  row j (raw) is published in a double-buffered row,
-//                      R[j][k] = S[j][k] / sqrt(S[j][j]) goes to the packed
-//                      factor in LDS, S[i][k] -= (S[j][i] / S[j][j]) S[j][k]
-//                      for j < i <= k in registers;
-//   inverse, step r (descending):  X[r][c] = T[r][c] / R[r][r] is published
-//                      (entries c < r of the buffer stay zero),
-//                      T[i][c] -= R[i][r] X[r][c] for i < r,
-// with T = I initially, so X = R^-1 row by row.  Dead pivots give zero rows
-// of R and zero rows and columns of X, as in chol_rinv_kernel.
-constexpr int CR_WAVES = 8;
-constexpr int CR_NT = 64 * CR_WAVES;
-constexpr int kCrMaxQ = 173;                                   // packed factor (<= 120 KB) in LDS
-constexpr int CR_ROWS = (kCrMaxQ + CR_WAVES - 1) / CR_WAVES;   // row slots per wave (<= 24)
-constexpr int CR_SLOTS = (kCrMaxQ + 63) / 64;                  // column slots per row
-constexpr int kCrBuf = kCrMaxQ + 64 * CR_SLOTS;                // row buffers, zero past q
-// row slot r holds rows i >= CR_WAVES r, so it needs at most cr_slots(r) column slots
-__host__ __device__ constexpr int cr_slots(int r) { return (kCrMaxQ - CR_WAVES * r + 63) / 64; }
-__host__ __device__ constexpr int cr_off(int r) { return r == 0 ? 0 : cr_off(r - 1) + cr_slots(r - 1); }
-constexpr int CR_REGS = cr_off(CR_ROWS);
-static_assert(CR_ROWS <= 24, "chol_rinv_reg: switch covers 24 row slots");
-
-// v[row slot R] -= a * buf[i + lane + 64 s] for every column slot
-template <int R>
-__device__ __forceinline__ void cr_update(double (&v)[CR_REGS], double a, const double* b) {
-  if constexpr (R < CR_ROWS) {
-#pragma unroll
-    for (int sl = 0; sl < cr_slots(R); ++sl) v[cr_off(R) + sl] -= a * b[64 * sl];
-  }
-}
-// copy row slot R to/from the row buffer (columns < q only), scaling by f
-template <int R>
-__device__ __forceinline__ void cr_publish(double (&v)[CR_REGS], double f, double* b, int k0,
-                                           int q) {
-  if constexpr (R < CR_ROWS) {
-#pragma unroll
-    for (int sl = 0; sl < cr_slots(R); ++sl) {
-      v[cr_off(R) + sl] *= f;
-      if (k0 + 64 * sl < q) b[64 * sl] = v[cr_off(R) + sl];
-    }
-  }
-}
-template <int R>
-__device__ __forceinline__ void cr_factor_row(const double (&v)[CR_REGS], double rjj, double irjj,
-                                              double* RS, int j, int lane, int q) {
-  if constexpr (R < CR_ROWS) {
-#pragma unroll
-    for (int sl = 0; sl < cr_slots(R); ++sl) {
-      const int k = j + lane + 64 * sl;
-      if (k < q) RS[pk(j, k)] = (k == j) ? rjj : v[cr_off(R) + sl] * irjj;
-    }
-  }
-}
-
-__global__ __launch_bounds__(CR_NT) void chol_rinv_reg_kernel(const double* __restrict__ G,
-                                                              double* __restrict__ Rinv, int q,
-                                                              double shift_rel, double rank_tol,
-                                                              int* dead_out,
-                                                              const int* only_if_dead) {
-  extern __shared__ double RS[];  // packed upper factor, pk(i, k)
-  __shared__ double buf[2][kCrBuf];
-  __shared__ int dead[kCrMaxQ + 4];
-  __shared__ double s_red[CR_WAVES][2];
-  __shared__ int any;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (only_if_dead) {
-    if (tid == 0) any = 0;
-    __syncthreads();
-    for (int i = tid; i < q; i += CR_NT)
-      if (only_if_dead[i]) any = 1;
-    __syncthreads();
-    if (!any) {
-      for (int e = tid; e < q * q; e += CR_NT) Rinv[e] = (e / q == e % q) ? 1.0 : 0.0;
-      return;
-    }
-  }
-  for (int e = tid; e < 2 * kCrBuf; e += CR_NT) (&buf[0][0])[e] = 0.0;
-  double tr = 0.0, dm = 0.0;
-  for (int i = tid; i < q; i += CR_NT) {
-    const double gii = G[(int64_t)i * q + i];
-    tr += gii;
-    dm = fmax(dm, gii);
-  }
-  for (int o = 32; o > 0; o >>= 1) { tr += __shfl_down(tr, o); dm = fmax(dm, __shfl_down(dm, o)); }
-  if (lane == 0) { s_red[wave][0] = tr; s_red[wave][1] = dm; }
-  __syncthreads();
-  if (tid == 0) {
-    double t0 = 0.0, t1 = 0.0;
-    for (int w = 0; w < CR_WAVES; ++w) { t0 += s_red[w][0]; t1 = fmax(t1, s_red[w][1]); }
-    s_red[0][0] = t0; s_red[0][1] = t1;
-  }
-  __syncthreads();
-  const double shift = shift_rel * s_red[0][0];
-  const double thresh = rank_tol * (s_red[0][1] + shift);
-
-  // v[cr_off(r) + s] = entry (i, k), i = wave + CR_WAVES r, k = i + lane + 64 s
-  double v[CR_REGS];
-#pragma unroll
-  for (int r = 0; r < CR_ROWS; ++r) {
-    const int i = wave + CR_WAVES * r;
-#pragma unroll
-    for (int sl = 0; sl < cr_slots(r); ++sl) {
-      const int k = i + lane + 64 * sl;
-      v[cr_off(r) + sl] = (i < q && k < q) ? G[(int64_t)i * q + k] + (k == i ? shift : 0.0) : 0.0;
-    }
-  }
-
-  for (int j = 0; j < q; ++j) {
-    double* rb = buf[j & 1];
-    const int jr = j / CR_WAVES;
-    const bool owner = (j % CR_WAVES) == wave;
-    if (owner) {
-      switch (jr) {
-#define CR_CASE(R) \
-  case R: cr_publish<R>(v, 1.0, rb + j + lane, j + lane, q); break;
-    CR_CASE(0)
-    CR_CASE(1)
-    CR_CASE(2)
-    CR_CASE(3)
-    CR_CASE(4)
-    CR_CASE(5)
-    CR_CASE(6)
-    CR_CASE(7)
-    CR_CASE(8)
-    CR_CASE(9)
-    CR_CASE(10)
-    CR_CASE(11)
-    CR_CASE(12)
-    CR_CASE(13)
-    CR_CASE(14)
-    CR_CASE(15)
-    CR_CASE(16)
-    CR_CASE(17)
-    CR_CASE(18)
-    CR_CASE(19)
-    CR_CASE(20)
-    CR_CASE(21)
-    CR_CASE(22)
-    CR_CASE(23)
-#undef CR_CASE
-        default: break;
-      }
-    }
-    __syncthreads();
-    const double d = rb[j];
-    const bool is_dead = !(d > thresh) || !isfinite(d);
-    if (owner) {
-      if (lane == 0) dead[j] = is_dead ? 1 : 0;
-      const double rjj = is_dead ? 0.0 : sqrt(d);
-      const double irjj = is_dead ? 0.0 : 1.0 / rjj;
-      switch (jr) {
-#define CR_CASE(R) \
-  case R: cr_factor_row<R>(v, rjj, irjj, RS, j, lane, q); break;
-    CR_CASE(0)
-    CR_CASE(1)
-    CR_CASE(2)
-    CR_CASE(3)
-    CR_CASE(4)
-    CR_CASE(5)
-    CR_CASE(6)
-    CR_CASE(7)
-    CR_CASE(8)
-    CR_CASE(9)
-    CR_CASE(10)
-    CR_CASE(11)
-    CR_CASE(12)
-    CR_CASE(13)
-    CR_CASE(14)
-    CR_CASE(15)
-    CR_CASE(16)
-    CR_CASE(17)
-    CR_CASE(18)
-    CR_CASE(19)
-    CR_CASE(20)
-    CR_CASE(21)
-    CR_CASE(22)
-    CR_CASE(23)
-#undef CR_CASE
-        default: break;
-      }
-    }
-    if (is_dead) continue;
-    const double invd = 1.0 / d;
-    // active rows: i = wave + CR_WAVES r > j, i.e. r >= r0
-    const int rs0 = (j + CR_WAVES - wave) / CR_WAVES;  // smallest r with wave + CR_WAVES r > j
-    switch (rs0) {
-#define CR_CASE(R)                                                                   \
-  case R:                                                                            \
-    if constexpr (R < CR_ROWS) {                                                     \
-      const int i = wave + CR_WAVES * R;                                             \
-      if (i < q) cr_update<R>(v, rb[i] * invd, rb + i + lane);                       \
-    }                                                                                \
-    [[fallthrough]];
-    CR_CASE(0)
-    CR_CASE(1)
-    CR_CASE(2)
-    CR_CASE(3)
-    CR_CASE(4)
-    CR_CASE(5)
-    CR_CASE(6)
-    CR_CASE(7)
-    CR_CASE(8)
-    CR_CASE(9)
-    CR_CASE(10)
-    CR_CASE(11)
-    CR_CASE(12)
-    CR_CASE(13)
-    CR_CASE(14)
-    CR_CASE(15)
-    CR_CASE(16)
-    CR_CASE(17)
-    CR_CASE(18)
-    CR_CASE(19)
-    CR_CASE(20)
-    CR_CASE(21)
-    CR_CASE(22)
-    CR_CASE(23)
-#undef CR_CASE
-      default: break;
-    }
-  }
-  __syncthreads();
-  if (dead_out) {
-    int anyd = 0;
-    for (int i = tid; i < q; i += CR_NT) {
-      dead_out[i] = dead[i];
-      anyd |= dead[i];
-    }
-    anyd = __syncthreads_or(anyd);
-    if (tid == 0) dead_out[q] = anyd;
-  }
-  for (int e = tid; e < 2 * kCrBuf; e += CR_NT) (&buf[0][0])[e] = 0.0;
-#pragma unroll
-  for (int r = 0; r < CR_ROWS; ++r)
-#pragma unroll
-    for (int sl = 0; sl < cr_slots(r); ++sl) v[cr_off(r) + sl] = (lane == 0 && sl == 0) ? 1.0 : 0.0;
-  __syncthreads();
-
-  for (int rr = q - 1; rr >= 0; --rr) {
-    double* xb = buf[rr & 1];
-    const int jr = rr / CR_WAVES;
-    if ((rr % CR_WAVES) == wave) {
-      const double inv = dead[rr] ? 0.0 : 1.0 / RS[pk(rr, rr)];
-      switch (jr) {
-#define CR_CASE(R) \
-  case R: cr_publish<R>(v, inv, xb + rr + lane, rr + lane, q); break;
-    CR_CASE(0)
-    CR_CASE(1)
-    CR_CASE(2)
-    CR_CASE(3)
-    CR_CASE(4)
-    CR_CASE(5)
-    CR_CASE(6)
-    CR_CASE(7)
-    CR_CASE(8)
-    CR_CASE(9)
-    CR_CASE(10)
-    CR_CASE(11)
-    CR_CASE(12)
-    CR_CASE(13)
-    CR_CASE(14)
-    CR_CASE(15)
-    CR_CASE(16)
-    CR_CASE(17)
-    CR_CASE(18)
-    CR_CASE(19)
-    CR_CASE(20)
-    CR_CASE(21)
-    CR_CASE(22)
-    CR_CASE(23)
-#undef CR_CASE
-        default: break;
-      }
-    }
-    __syncthreads();
-    if (dead[rr]) continue;
-    // active rows: i = wave + CR_WAVES r < rr, i.e. r < re
-    const int re = rr > wave ? (rr - wave + CR_WAVES - 1) / CR_WAVES : 0;
-    switch (re) {
-#define CR_CASE(R)                                                                   \
-  case R + 1:                                                                        \
-    if constexpr (R < CR_ROWS) {                                                     \
-      const int i = wave + CR_WAVES * R;                                             \
-      cr_update<R>(v, RS[pk(i, rr)], xb + i + lane);                                 \
-    }                                                                                \
-    [[fallthrough]];
-      CR_CASE(23) CR_CASE(22) CR_CASE(21) CR_CASE(20) CR_CASE(19) CR_CASE(18) CR_CASE(17)
-      CR_CASE(16) CR_CASE(15) CR_CASE(14) CR_CASE(13) CR_CASE(12) CR_CASE(11) CR_CASE(10)
-      CR_CASE(9) CR_CASE(8) CR_CASE(7) CR_CASE(6) CR_CASE(5) CR_CASE(4) CR_CASE(3)
-      CR_CASE(2) CR_CASE(1) CR_CASE(0)
-#undef CR_CASE
-      default: break;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < CR_ROWS; ++r) {
-    const int i = wave + CR_WAVES * r;
-    if (i >= q) continue;
-#pragma unroll
-    for (int sl = 0; sl < cr_slots(r); ++sl) {
-      const int k = i + lane + 64 * sl;
-      if (k < q) Rinv[(int64_t)i * q + k] = v[cr_off(r) + sl];
-    }
-  }
-  for (int e = tid; e < q * q; e += CR_NT)
-    if (e / q > e % q) Rinv[e] = 0.0;
-}
-
 // Blocked variant on the FP64 matrix cores for q <= kCbMaxQ: the matrix is
 // held as its upper 16 x 16 tiles in LDS (rows padded to 17 doubles), and
 //   for each tile column k:  one wave factors the diagonal tile (element
@@ -1015,6 +699,279 @@ __global__ __launch_bounds__(64 * kRepWaves) void jacobi_vectors_kernel(
 }
 
 // --------------------------------------------------------------------------
+// Block Jacobi for q <= kBjMaxQ.  The indices are grouped in blocks of 8
+// (N blocks, N even, padding indices never rotate); a block round pairs the
+// blocks by the same round-robin order, and for each block pair (I, J) one
+// wave rotates the 16 x 16 subproblem [[S_II, S_IJ], [S_JI, S_JJ]] in place
+// (the first block round of a sweep runs all 15 inner rounds, the others only
+// the 8 rounds of cross pairs (p in I, q in J), so every index pair is
+// rotated once or more per sweep), accumulating V_a = product of its
+// rotations.  The rest of the matrix follows on the FP64 matrix cores: every
+// off-diagonal 16 x 16 tile of two block pairs (a, c) becomes
+// V_a^T S_ac V_c.  A sweep is (N - 1) block rounds with two barriers each,
+// against n - 1 element rounds; the V_a of every block round are recorded
+// and replayed on V by row tiles with the same matrix-core products.
+// S holds the upper triangle as 8 x 8 blocks (diagonal blocks use their upper
+// half) in LDS.
+// --------------------------------------------------------------------------
+constexpr int kBjMaxQ = 160;
+constexpr int kBjMaxPairs = kBjMaxQ / 16;   // block pairs per block round (N / 2)
+constexpr int BJ_NT = 64 * kBjMaxPairs;
+constexpr int BJ_LD = 17;
+constexpr int kBjMaxSweeps = 40;
+
+__device__ __forceinline__ int bj_addr(int i, int j) {
+  if (i > j) { const int t = i; i = j; j = t; }
+  const int I = i >> 3, J = j >> 3;
+  return (J * (J + 1) / 2 + I) * 64 + (i & 7) * 8 + (j & 7);
+}
+__device__ __forceinline__ int bj_global(int li, int I, int J) {
+  return li < 8 ? 8 * I + li : 8 * J + li - 8;
+}
+
+__global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restrict__ Gin, int q,
+                                                           double* __restrict__ w,
+                                                           double* __restrict__ vrec, int* info,
+                                                           double tol) {
+  extern __shared__ double S[];  // 8 x 8 blocks of the upper triangle
+  __shared__ double Vs[kBjMaxPairs][16 * BJ_LD];
+  __shared__ double Ws[kBjMaxPairs][16 * BJ_LD];
+  __shared__ double csw[kBjMaxPairs][8][2];
+  __shared__ int pqw[kBjMaxPairs][8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nb = (q + 7) / 8;
+  const int N = nb + (nb & 1);
+  const int NP = N / 2;
+  const int nstore = N * (N + 1) / 2 * 64;
+  for (int e = tid; e < nstore; e += BJ_NT) S[e] = 0.0;
+  __syncthreads();
+  for (int e = tid; e < q * q; e += BJ_NT) {
+    const int i = e / q, j = e % q;
+    if (i <= j) S[bj_addr(i, j)] = 0.5 * (Gin[(int64_t)i * q + j] + Gin[(int64_t)j * q + i]);
+  }
+  // this lane's 2 x 2 block (ka <= kb) of the 8 inner pairs (lanes < 36)
+  int ka = 0, kb = 0;
+  {
+    int k = 0;
+    while ((k + 1) * (k + 2) / 2 <= lane) ++k;
+    kb = k;
+    ka = lane - k * (k + 1) / 2;
+  }
+  const int vr = lane >> 2, vk = 2 * (lane & 3);  // V_a row and first pair of this lane
+  const int ntile = NP * (NP - 1) / 2;
+  __syncthreads();
+
+  int sweep = 0;
+  for (; sweep < kBjMaxSweeps; ++sweep) {
+    int need = 0;
+    for (int j = tid >> 3; j < q; j += BJ_NT / 8) {
+      const double ajj = S[bj_addr(j, j)];
+      for (int i = tid & 7; i < j && !need; i += 8) {
+        const double aij = S[bj_addr(i, j)];
+        need = aij != 0.0 && aij * aij > tol * tol * fabs(S[bj_addr(i, i)] * ajj);
+      }
+    }
+    if (!__syncthreads_or(need)) break;
+    for (int r = 0; r < N - 1; ++r) {
+      // ---- phase 1: wave a rotates the subproblem of block pair a ----
+      // (on a dense symmetric copy W, element (x, y) at min * BJ_LD + max)
+      if (wave < NP) {
+        int I, J;
+        round_pair(N, r, wave, I, J);
+        double* V = Vs[wave];
+        double* W = Ws[wave];
+        for (int e = lane; e < 256; e += 64) {
+          const int x = e >> 4, y = e & 15;
+          if (x <= y) W[x * BJ_LD + y] = S[bj_addr(bj_global(x, I, J), bj_global(y, I, J))];
+          V[x * BJ_LD + y] = (x == y) ? 1.0 : 0.0;
+        }
+        // local indices >= nI (block I) or >= 8 + nJ (block J) are padding
+        const int nI = min(8, max(0, q - 8 * I)), nJ = min(8, max(0, q - 8 * J));
+        rep_wave_sync();
+        const int inner = (r == 0) ? 15 : 8;
+        for (int rr = 0; rr < inner; ++rr) {
+          if (lane < 8) {
+            int p, qq;
+            if (r == 0) round_pair(16, rr, lane, p, qq);
+            else { p = lane; qq = 8 + ((lane + rr) & 7); }
+            double c = 1.0, sn = 0.0;
+            const bool live = (p < 8 ? p < nI : p - 8 < nJ) && (qq < 8 ? qq < nI : qq - 8 < nJ);
+            if (live) {
+              const double app = W[p * (BJ_LD + 1)], aqq = W[qq * (BJ_LD + 1)], apq = W[p * BJ_LD + qq];
+              if (apq != 0.0 && apq * apq > tol * tol * fabs(app * aqq)) jacobi_rotation(app, aqq, apq, c, sn);
+            }
+            csw[wave][lane][0] = c;
+            csw[wave][lane][1] = sn;
+            pqw[wave][lane] = p | (qq << 8);
+          }
+          rep_wave_sync();
+          if (lane < 36) {
+            const double ca = csw[wave][ka][0], sa = csw[wave][ka][1];
+            const double cb = csw[wave][kb][0], sb = csw[wave][kb][1];
+            const int pqa = pqw[wave][ka], pqb = pqw[wave][kb];
+            const int pa = pqa & 255, qa = pqa >> 8, pb = pqb & 255, qb = pqb >> 8;
+            if (ka == kb) {
+              if (sa != 0.0) {
+                const int ipp = pa * (BJ_LD + 1), iqq = qa * (BJ_LD + 1), ipq = pa * BJ_LD + qa;
+                const double app = W[ipp], aqq = W[iqq], apq = W[ipq];
+                const double t = sa / ca;
+                W[ipp] = app - t * apq;
+                W[iqq] = aqq + t * apq;
+                W[ipq] = 0.0;
+              }
+            } else if (sa != 0.0 || sb != 0.0) {
+              const int i11 = min(pa, pb) * BJ_LD + max(pa, pb), i12 = min(pa, qb) * BJ_LD + max(pa, qb);
+              const int i21 = min(qa, pb) * BJ_LD + max(qa, pb), i22 = min(qa, qb) * BJ_LD + max(qa, qb);
+              const double x11 = W[i11], x12 = W[i12], x21 = W[i21], x22 = W[i22];
+              const double y11 = x11 * cb - x12 * sb, y12 = x11 * sb + x12 * cb;
+              const double y21 = x21 * cb - x22 * sb, y22 = x21 * sb + x22 * cb;
+              W[i11] = ca * y11 - sa * y21;
+              W[i12] = ca * y12 - sa * y22;
+              W[i21] = sa * y11 + ca * y21;
+              W[i22] = sa * y12 + ca * y22;
+            }
+          }
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const double c = csw[wave][vk + h][0], sn = csw[wave][vk + h][1];
+            if (sn != 0.0) {
+              const int pq = pqw[wave][vk + h];
+              double* vrow = V + vr * BJ_LD;
+              const double vp = vrow[pq & 255], vq = vrow[pq >> 8];
+              vrow[pq & 255] = c * vp - sn * vq;
+              vrow[pq >> 8] = sn * vp + c * vq;
+            }
+          }
+          rep_wave_sync();
+        }
+        for (int e = lane; e < 256; e += 64) {
+          const int x = e >> 4, y = e & 15;
+          if (x <= y) S[bj_addr(bj_global(x, I, J), bj_global(y, I, J))] = W[x * BJ_LD + y];
+        }
+        double* out = vrec + ((int64_t)(sweep * (N - 1) + r) * NP + wave) * 256;
+        for (int e = lane; e < 256; e += 64) out[e] = V[(e >> 4) * BJ_LD + (e & 15)];
+      }
+      __syncthreads();
+      // ---- phase 2: S_ac <- V_a^T S_ac V_c for every pair of block pairs a < c ----
+      for (int u = wave; u < ntile; u += kBjMaxPairs) {
+        int c = 1;
+        while (c * (c + 1) / 2 <= u) ++c;
+        const int a = u - c * (c - 1) / 2;
+        int Ia, Ja, Ic, Jc;
+        round_pair(N, r, a, Ia, Ja);
+        round_pair(N, r, c, Ic, Jc);
+        const int m = lane & 15, kq = lane >> 4;
+        // element (x, y) of the tile lives in block (x < 8 ? Ia : Ja, y < 8 ? Ic : Jc);
+        // per lane, the two blocks of its row m (loads) and of its column m (stores)
+        int lb[2], sb[2];
+        bool lt[2], st[2];
+        {
+          const int bx = m < 8 ? Ia : Ja, by = m < 8 ? Ic : Jc;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int cy = h ? Jc : Ic, rx = h ? Ja : Ia;
+            lt[h] = bx > cy;
+            lb[h] = lt[h] ? (bx * (bx + 1) / 2 + cy) * 64 + (m & 7) : (cy * (cy + 1) / 2 + bx) * 64 + (m & 7) * 8;
+            st[h] = rx > by;
+            sb[h] = st[h] ? (rx * (rx + 1) / 2 + by) * 64 + (m & 7) * 8 : (by * (by + 1) / 2 + rx) * 64 + (m & 7);
+          }
+        }
+        cb_d4 T = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const int k = 4 * ks + kq, h = ks >> 1, yi = k & 7;
+          const double av = S[lb[h] + (lt[h] ? yi * 8 : yi)];
+          const double bv = Vs[c][k * BJ_LD + m];
+          T = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, T, 0, 0, 0);
+        }
+        cb_d4 R = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const double av = Vs[a][(4 * ks + kq) * BJ_LD + m];
+          R = __builtin_amdgcn_mfma_f64_16x16x4f64(av, T[ks], R, 0, 0, 0);
+        }
+        rep_wave_sync();
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+          const int xi = (kq + 4 * r4) & 7, h = r4 >> 1;
+          S[sb[h] + (st[h] ? xi : xi * 8)] = R[r4];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) info[0] = sweep;
+  for (int i = tid; i < q; i += BJ_NT) w[i] = S[bj_addr(i, i)];
+}
+
+// Replay of the recorded block rotations on V, one workgroup per 16-row tile
+// (rows held in LDS); the V_a of the next block round are loaded into
+// registers while the current one is applied.  Then the eigenpairs are
+// sorted by descending eigenvalue (ties: lower index first).
+__global__ __launch_bounds__(BJ_NT) void jacobi_blk_vectors_kernel(
+    const double* __restrict__ vrec, const int* __restrict__ info, int q, double* __restrict__ w,
+    double* __restrict__ V, const double* __restrict__ wraw) {
+  __shared__ double Vt[16][kBjMaxQ + 1];
+  __shared__ int perm[kBjMaxQ];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nb = (q + 7) / 8;
+  const int N = nb + (nb & 1);
+  const int NP = N / 2;
+  const int P = 8 * N;
+  const int row0 = blockIdx.x * 16;
+  for (int e = tid; e < 16 * P; e += BJ_NT) {
+    const int i = e / P, j = e % P;
+    Vt[i][j] = (row0 + i == j) ? 1.0 : 0.0;
+  }
+  const int nrounds = info[0] * (N - 1);
+  const int m = lane & 15, kq = lane >> 4;
+  double bnext[4] = {0.0, 0.0, 0.0, 0.0};
+  if (wave < NP && nrounds > 0) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) bnext[ks] = vrec[(int64_t)wave * 256 + (4 * ks + kq) * 16 + m];
+  }
+  for (int t = 0; t < nrounds; ++t) {
+    __syncthreads();
+    if (wave < NP) {
+      double b[4];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) b[ks] = bnext[ks];
+      if (t + 1 < nrounds) {
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+          bnext[ks] = vrec[((int64_t)(t + 1) * NP + wave) * 256 + (4 * ks + kq) * 16 + m];
+      }
+      int I, J;
+      round_pair(N, t % (N - 1), wave, I, J);
+      cb_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vt[m][bj_global(4 * ks + kq, I, J)], b[ks], acc, 0, 0, 0);
+      rep_wave_sync();
+      const int gn = bj_global(m, I, J);
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) Vt[kq + 4 * r4][gn] = acc[r4];
+    }
+  }
+  for (int i = tid; i < q; i += BJ_NT) {
+    const double wi = wraw[i];
+    int rank = 0;
+    for (int j = 0; j < q; ++j) {
+      const double wj = wraw[j];
+      rank += (wj > wi) || (wj == wi && j < i);
+    }
+    perm[rank] = i;
+  }
+  __syncthreads();
+  for (int e = tid; e < 16 * q; e += BJ_NT) {
+    const int i = e / q, pos = e % q;
+    if (row0 + i < q) V[(int64_t)(row0 + i) * q + pos] = Vt[i][perm[pos]];
+  }
+  if (blockIdx.x == 0)
+    for (int i = tid; i < q; i += BJ_NT) w[i] = wraw[perm[i]];
+}
+
+// --------------------------------------------------------------------------
 __global__ void count_kept_kernel(const double* __restrict__ w, int q, int chi, double cutoff,
                                   double* __restrict__ s_out, int* cD_out) {
   __shared__ int cnt;
@@ -1149,18 +1106,49 @@ void refill_dead_columns(double* Y, int64_t m, int q, const int* dead, uint64_t 
   ACETN_HIP_CHECK(hipGetLastError());
 }
 
+// recorded rotations: element rounds ((c, s) per pair) or, for q <= kBjMaxQ,
+// block rounds (a 16 x 16 V_a per block pair)
+size_t jacobi_record_elems(int q) {
+  const int n = q + (q & 1);
+  size_t e = (size_t)kMaxSweeps * (n - 1) * (n / 2) * 2;
+  if (q <= kBjMaxQ) {
+    const int nb = (q + 7) / 8, N = nb + (nb & 1);
+    e = std::max(e, (size_t)kBjMaxSweeps * (N - 1) * (N / 2) * 256);
+  }
+  return e;
+}
+
 size_t jacobi_rot_elems(int q) {
   const int n = q + (q & 1);
   // rotations + raw eigenvalues (q) + info (as doubles) + packed scratch
-  return (size_t)kMaxSweeps * (n - 1) * (n / 2) * 2 + (size_t)q + 8 + (size_t)n * (n + 1) / 2;
+  return jacobi_record_elems(q) + (size_t)q + 8 + (size_t)n * (n + 1) / 2;
 }
 
 void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s, double tol) {
   ACETN_REQUIRE(q >= 1 && q <= 510, "jacobi_eigh: q out of range");
   const int n = q + (q & 1);
-  double* wraw = rot + (size_t)kMaxSweeps * (n - 1) * (n / 2) * 2;
+  double* wraw = rot + jacobi_record_elems(q);
   int* info = reinterpret_cast<int*>(wraw + q);
   double* gscr = wraw + q + 8;
+  if (q <= kBjMaxQ) {
+    const int nb = (q + 7) / 8, N = nb + (nb & 1);
+    const size_t lds = sizeof(double) * (size_t)N * (N + 1) / 2 * 64;
+    static bool attr = false;
+    if (!attr) {
+      // the largest block store (N = kBjMaxQ / 8 blocks) next to the static LDS
+      constexpr int kNmax = kBjMaxQ / 8;
+      ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)jacobi_blk_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(sizeof(double) * kNmax * (kNmax + 1) / 2 * 64)));
+      attr = true;
+    }
+    hipLaunchKernelGGL(jacobi_blk_kernel, dim3(1), dim3(BJ_NT), lds, s, G, q, wraw, rot, info, tol);
+    ACETN_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(jacobi_blk_vectors_kernel, dim3((8 * N + 15) / 16), dim3(BJ_NT), 0, s, rot,
+                       info, q, w, V, wraw);
+    ACETN_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const size_t pb = packed_bytes(n);
   const int use_lds = pb <= kLdsBudget;
   const int half = n / 2;

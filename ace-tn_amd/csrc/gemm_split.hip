@@ -63,6 +63,52 @@ __global__ void split_planes_kernel(const double* __restrict__ B, IdxMap bk, Idx
   }
 }
 
+// The same planes when n is the unit-stride index of X (k strided): 64 x 64
+// tiles transposed through LDS, so reads run along n and writes along k.
+template <int SPLIT>
+__global__ __launch_bounds__(256) void split_planes_t_kernel(const double* __restrict__ B, IdxMap bk,
+                                                             IdxMap bn, int K, int N, int Kp,
+                                                             __bf16* __restrict__ planes) {
+  __shared__ float tile[64][65];
+  const int k0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int64_t total = (int64_t)N * Kp;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int kk = e / 64, nn = e % 64;
+    const int k = k0 + kk, n = n0 + nn;
+    tile[kk][nn] = (k < K && n < N) ? (float)B[imap(bk, k) + imap(bn, n)] : 0.f;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int nn = e / 64, kk = e % 64;
+    const int k = k0 + kk, n = n0 + nn;
+    if (n < N && k < Kp) {
+      __bf16 pc[SPLIT];
+      split<SPLIT>(tile[kk][nn], pc);
+#pragma unroll
+      for (int s = 0; s < SPLIT; ++s) planes[(int64_t)s * total + (int64_t)n * Kp + k] = pc[s];
+    }
+  }
+}
+
+// planes[s][n][k] of X(k, n) = X[km(k) + nm(n)], the kernel chosen by which
+// index is unit-stride
+template <int SPLIT>
+void split_into_planes(const double* X, const IdxMap& km, const IdxMap& nm, int K, int N, int Kp,
+                       __bf16* planes, hipStream_t s) {
+  if (nm.s0 == 1 && km.s0 != 1) {
+    dim3 grid((Kp + 63) / 64, (N + 63) / 64);
+    ACETN_REQUIRE(grid.y <= 65535, "split_planes: too many rows");
+    hipLaunchKernelGGL(split_planes_t_kernel<SPLIT>, grid, dim3(256), 0, s, X, km, nm, K, N, Kp,
+                       planes);
+  } else {
+    const int64_t total = (int64_t)N * Kp;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(split_planes_kernel<SPLIT>, dim3(blocks), dim3(256), 0, s, X, km, nm, K, N,
+                       Kp, planes);
+  }
+  ACETN_HIP_CHECK(hipGetLastError());
+}
+
 // X (R x Cc, row-major, leading dimension ld) -> planes of both orientations:
 // PN[s][r][c] (Cp columns, zero padded) and PT[s][c][r] (Rp columns).  One
 // 64 x 64 tile per block, transposed through LDS.
@@ -571,13 +617,7 @@ void run(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, c
          double* C, int ks, double* work, hipStream_t s) {
   const int Kp = padded_k(g.K);
   __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, ks));
-  {
-    const int64_t total = (int64_t)g.N * Kp;
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
-    hipLaunchKernelGGL(split_planes_kernel<SPLIT>, dim3(blocks), dim3(256), 0, s, B, g.bk, g.bn,
-                       g.K, g.N, Kp, planes);
-    ACETN_HIP_CHECK(hipGetLastError());
-  }
+  split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
   const Tile t = choose_tile(g);
   if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
   else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
@@ -586,9 +626,14 @@ void run(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, c
 }
 }  // namespace
 
+// Large general products split A into planes once too (one extra launch,
+// no per-tile conversion); small ones convert in the 8-wave kernel.
+constexpr double kPlanarFlops = 1e8;
+
 size_t gemm_split_work_elems(const GemmDesc& g, int count) {
   ACETN_REQUIRE(count == 1, "gemm_split: batched products are not supported");
-  return std::max(partial_elems(g, choose_ksplit(g)) + plane_elems(g, 3),
+  return std::max(partial_elems(g, choose_ksplit(g)) + plane_elems(g, 3) +
+                      split_planes_elems(g.M, g.K),
                   general_partial_elems(g, general_ksplit(g)));
 }
 
@@ -600,6 +645,22 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
                 "gemm_split: FP64 operands and result only");
   if (g.M <= 0 || g.N <= 0) return;
   ACETN_REQUIRE(g.K > 0 || g.beta == 0.0, "gemm_split: K == 0 with beta != 0 unsupported");
+  if (2.0 * g.M * g.N * (double)g.K >= kPlanarFlops) {
+    const size_t apl = split_planes_elems(g.M, g.K);
+    int ks = choose_ksplit(g);
+    if (work_elems < partial_elems(g, ks) + plane_elems(g, 3) + apl) ks = 1;
+    ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, ks) + plane_elems(g, 3) + apl,
+                  "gemm_split: scratch too small");
+    const int Kp = padded_k(g.K);
+    __bf16* Ap = reinterpret_cast<__bf16*>(work + partial_elems(g, ks) + plane_elems(g, 3));
+    const int64_t total = (int64_t)g.M * Kp;
+    // planes[s][m][k] of A(m, k) = A[am(m) + ak(k)]: the B-plane split with the maps swapped
+    if (split == 2) split_into_planes<2>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s);
+    else split_into_planes<3>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s);
+    if (split == 2) run<2, true>(g, nullptr, Ap, total, b.B[0], b.C[0], ks, work, s);
+    else run<3, true>(g, nullptr, Ap, total, b.B[0], b.C[0], ks, work, s);
+    return;
+  }
   int ks = general_ksplit(g);
   if (work == nullptr || work_elems < general_partial_elems(g, ks)) ks = 1;
   if (split == 2) run_general<2>(g, b.A[0], b.B[0], b.C[0], ks, work, s);

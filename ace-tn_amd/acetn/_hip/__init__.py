@@ -231,8 +231,13 @@ def serialize(flag):
 
 
 def slot_stream(slot, device):
-    """Default-priority side streams, one per concurrent slot."""
-    if _serial:
+    """Default-priority streams, one per concurrent slot.  Slot 0 runs on the
+    caller's current stream: the runtime spreads streams over a small fixed
+    number of hardware queues (4 by default), and the current stream plus
+    three side streams land on four distinct queues, where the current
+    stream and four side streams put two slots on one queue and serialise
+    them."""
+    if _serial or slot == 0:
         return torch.cuda.current_stream(device)
     key = (slot, str(device))
     if key not in _streams:

@@ -415,9 +415,12 @@ class ProjectorJob:
                 self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
                 self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec), ptr(self.ws),
                 self.ws.numel(), c_vp(self.stream.cuda_stream)))
+            # marks the truncation count's arrival in pinned host memory
+            self.ready = torch.cuda.Event()
+            self.ready.record(self.stream)
 
     def finish(self):
-        """Call after the stream has been synchronised."""
+        """Call once ``ready`` has completed (or the stream was synchronised)."""
         n = int(self.cd_host[0])
         with torch.cuda.stream(self.stream):
             P1 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)
@@ -442,7 +445,9 @@ def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None, precision="fp6
     Eo = torch.empty(cDi, cDj, D, D, dtype=torch.float64, device=dev)
     P1i, P2i, P1j, P2j = (t.contiguous() for t in (P1i, P2i, P1j, P2j))
     nb = lib().acetn_hip_renormalize_workspace(ctypes.byref(s1), k, cDi, cDj)
-    ws = workspace(nb, dev) if slot is None else slot_workspace(slot, nb, dev)
+    # own per-slot scratch: a renormalisation may run while other slots'
+    # projector pairs still use theirs
+    ws = workspace(nb, dev) if slot is None else slot_workspace(("renormalize", slot), nb, dev)
     check(lib().acetn_hip_renormalize_boundary(ctypes.byref(s1), k, ptr(P1i), ptr(P2i), cDi,
                                                ptr(P1j), ptr(P2j), cDj, precision_code(precision),
                                                ptr(Ca), ptr(Cb), ptr(Eo),

@@ -43,6 +43,12 @@ class DirectionalMover:
         self._stage(ipeps, moves)
 
     def _stage(self, ipeps, moves):
+        """Projector pairs of every plaquette of the moves, then the
+        renormalisations.  A renormalisation starts as soon as both of its
+        projector pairs exist (its slot stream waits on their completion
+        events), so the first move's renormalisations overlap the later
+        pairs' projector work; the new boundary tensors are written back
+        after everything has finished."""
         import torch
         jobs, targets = [], []
         for line, k in moves:
@@ -54,13 +60,21 @@ class DirectionalMover:
             for idx in range(n_other):
                 s1, s2, i, j = self.targets(ipeps.nx, ipeps.ny, line, k, idx)
                 targets.append((s1, s2, base + i, base + j, k))
-        proj = self.projector_calculator.calculate_many(ipeps, jobs)
-        results = []
-        for slot, (s1, s2, i, j, k) in enumerate(targets):
-            stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
-            with torch.cuda.stream(stream):
-                results.append(self._renormalize(ipeps, proj[i], proj[j], s1, k, slot,
-                                                 self.projector_calculator.precision))
+        calc = self.projector_calculator
+        started = calc.start_many(ipeps, jobs)
+        proj, done = [None] * len(jobs), [None] * len(jobs)
+        results = [None] * len(targets)
+        for idx, pr, ev in calc.finish_each(ipeps, started):
+            proj[idx], done[idx] = pr, ev
+            for slot, (s1, s2, i, j, k) in enumerate(targets):
+                if results[slot] is not None or proj[i] is None or proj[j] is None:
+                    continue
+                stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
+                stream.wait_event(done[i])
+                stream.wait_event(done[j])
+                with torch.cuda.stream(stream):
+                    results[slot] = self._renormalize(ipeps, proj[i], proj[j], s1, k, slot,
+                                                      calc.precision)
         torch.cuda.synchronize()
         for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(targets, results):
             dst = ipeps[s2]

@@ -58,14 +58,21 @@ class ProjectorCalculator:
     def calculate_many(self, ipeps, jobs, system=None):
         """Projector pairs of mutually independent plaquettes, ``jobs`` a list
         of (sites, k) in the reference's call order (which fixes their sketch
-        streams).  Each pair runs on its own stream; one host synchronisation
-        reads all truncation ranks."""
+        streams).  Each pair runs on its own stream; the host reads each
+        truncation rank as soon as its pair has computed it."""
         import torch
+        out = list(self.finish_each(ipeps, self.start_many(ipeps, jobs, system)))
+        torch.cuda.synchronize()
+        return [pr for _, pr, _ in out]
+
+    def start_many(self, ipeps, jobs, system=None):
+        """Enqueue every projector pair up to its truncation count (one
+        stream per pair); returns the started jobs in ``jobs`` order."""
         system = system or ("full-system" if self.calculate == self.calculate_full_system
                             else "half-system")
         if self.svd_type not in ("rsvd", "full-rank"):
             raise ValueError(f"Invalid svd_type: {self.svd_type}")
-        chi, D = ipeps.dims['chi'], ipeps.dims['bond']
+        chi = ipeps.dims['chi']
         started = []
         for slot, (sites, k) in enumerate(jobs):
             sites = [tuple(s) for s in sites]
@@ -77,13 +84,22 @@ class ProjectorCalculator:
                                              self.svd_cutoff, self.sketch.seed, stream_id,
                                              precision=self.precision, system=system,
                                              svd=self.svd_type))
-        torch.cuda.synchronize()
-        out = []
-        for job in started:
+        return started
+
+    def finish_each(self, ipeps, started):
+        """Yields (index, (P1, P2), done_event) per started job, in order: the
+        host waits for that job's truncation count only, then enqueues its
+        projector products on the job's stream; ``done_event`` marks their
+        completion on that stream."""
+        import torch
+        D = ipeps.dims['bond']
+        for idx, job in enumerate(started):
+            job.ready.synchronize()
             P1, P2 = job.finish()
+            done = torch.cuda.Event()
+            done.record(job.stream)
             rows = P1.shape[0]
-            out.append((P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)))
-        self.last_spectrum = started[-1].spec
-        torch.cuda.synchronize()
-        return out
+            yield idx, (P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)), done
+        if started:
+            self.last_spectrum = started[-1].spec
 

@@ -94,7 +94,11 @@ __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; 
 // Workgroup tile BM x BN with BM = 32*WM, BN = 32*WN: 4 waves in a 2 x 2
 // grid, each wave owning WM x WN MFMA blocks of 16 x 16.  Double-buffered LDS
 // (one barrier per k-tile) with the next tile prefetched into registers.
-template <int WM, int WN>
+// LIN: every operand map is a single stride (no grouped digits), so an
+// element's address is base + row*stride + k*stride: rows past the edge are
+// clamped to the last row (their results are never stored) and only the K
+// edge needs a predicate, which keeps the tile loads free of branches.
+template <int WM, int WN, bool LIN>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, int ksplit,
                                                       double* __restrict__ work) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
@@ -128,7 +132,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
     else         { a_kl[i] = e % BK; a_ml[i] = e / BK; }
     const int m = m0 + a_ml[i];
     a_mok[i] = m < g.M;
-    a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
+    if (LIN) a_moff[i] = (int64_t)min(m, g.M - 1) * g.am.s0;
+    else a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
   }
   const bool b_nfast = contiguous(g.bn) || !contiguous(g.bk);
   int b_nl[BE], b_kl[BE];
@@ -141,7 +146,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
     else         { b_kl[i] = e % BK; b_nl[i] = e / BK; }
     const int n = n0 + b_nl[i];
     b_nok[i] = n < g.N;
-    b_noff[i] = b_nok[i] ? imap(g.bn, n) : 0;
+    if (LIN) b_noff[i] = (int64_t)min(n, g.N - 1) * g.bn.s0;
+    else b_noff[i] = b_nok[i] ? imap(g.bn, n) : 0;
   }
 
   d4 acc[WM][WN];
@@ -152,6 +158,22 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
 
   double ra[AE], rb[BE];
   auto load_tile = [&](int kt) {
+    if (LIN) {
+      const int64_t sak = g.ak.s0, sbk = g.bk.s0;
+#pragma unroll
+      for (int i = 0; i < AE; ++i) {
+        const int k = kt + a_kl[i];
+        const double v = A[a_moff[i] + (int64_t)min(k, kend - 1) * sak];
+        ra[i] = k < kend ? v : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < BE; ++i) {
+        const int k = kt + b_kl[i];
+        const double v = B[(int64_t)min(k, kend - 1) * sbk + b_noff[i]];
+        rb[i] = k < kend ? v : 0.0;
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
       const int k = kt + a_kl[i];
@@ -226,8 +248,18 @@ __global__ void splitk_reduce_kernel(GemmDesc g, GemmBatch bt, int ksplit,
   const int prob = blockIdx.y;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
        e += (int64_t)gridDim.x * blockDim.x) {
+    // eight partial loads in flight per step; the sum keeps the part order
+    const double* wp = work + (int64_t)prob * ksplit * mn + e;
     double s = 0.0;
-    for (int p = 0; p < ksplit; ++p) s += work[((int64_t)prob * ksplit + p) * mn + e];
+    int p = 0;
+    for (; p + 8 <= ksplit; p += 8) {
+      double v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = wp[(int64_t)(p + t) * mn];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) s += v[t];
+    }
+    for (; p < ksplit; ++p) s += wp[(int64_t)p * mn];
     const int m = (int)(e / g.N), n = (int)(e % g.N);
     double* C = bt.C[prob];
     const int64_t off = imap(g.cm, m) + imap(g.cn, n);
@@ -264,7 +296,9 @@ void launch_gemm(const GemmDesc& g, const GemmBatch& b, int ks, double* work, hi
   constexpr int BM = 32 * WM, BN = 32 * WN;
   dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, b.count * ks);
   ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_f64: grid too large");
-  hipLaunchKernelGGL((gemm_f64_kernel<WM, WN>), grid, dim3(NT), 0, s, g, b, ks, work);
+  const bool lin = g.am.n0 <= 0 && g.ak.n0 <= 0 && g.bk.n0 <= 0 && g.bn.n0 <= 0;
+  if (lin) hipLaunchKernelGGL((gemm_f64_kernel<WM, WN, true>), grid, dim3(NT), 0, s, g, b, ks, work);
+  else hipLaunchKernelGGL((gemm_f64_kernel<WM, WN, false>), grid, dim3(NT), 0, s, g, b, ks, work);
 }
 }  // namespace
 

@@ -34,6 +34,7 @@ typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 namespace {
 constexpr int BK = 32, NT = 256, KPAD = 8;
+constexpr int kCUs = 256;  // MI355X compute units
 
 __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; }
 
@@ -143,23 +144,31 @@ __global__ __launch_bounds__(256) void split_transpose_kernel(const double* __re
   }
 }
 
-template <int WM, int WN, int SPLIT, bool APL>
-__global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double* __restrict__ A,
-                                                        const __bf16* __restrict__ Ap, int64_t aplane,
+// Planar product: A and B both arrive as bf16 planes, A [s][m][Kp] and B
+// [s][n][Kp] (k padded to Kp with zeros), workgroup tile 32*WM x 32*WN,
+// 4 waves in a 2 x 2 grid.  Every lane moves whole 16-byte octets; rows past
+// the matrix edge are clamped to the last row (their results are never
+// stored), so the loads carry no predicates.  The next k-tile is in flight
+// in registers while the MFMAs run on the LDS tile (a second A tile in
+// flight measured no faster: the kernel is not load-latency bound).  Split-K
+// partials
+// are FP32, the accumulator type, so they lose nothing.  Offsets are 32-bit
+// (the host checks that every plane set stays below 2^31 elements).
+template <int WM, int WN, int SPLIT>
+__global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __bf16* __restrict__ Ap,
+                                                        int64_t aplane,
                                                         const __bf16* __restrict__ Bp, int Kp,
                                                         double* __restrict__ C, int ksplit,
-                                                        double* __restrict__ work) {
+                                                        float* __restrict__ work) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
-  constexpr int AE = BM * BK / NT;
   constexpr int LK = BK + KPAD;
-  constexpr int BOCT = SPLIT * BN * (BK / 8);          // 16-byte octets of B per k-tile
+  constexpr int BOCT = SPLIT * BN * (BK / 8);  // 16-byte octets of B per k-tile
   constexpr int BV = (BOCT + NT - 1) / NT;
-  constexpr int AOCT = SPLIT * BM * (BK / 8);          // same for planar A
-  constexpr int AV = APL ? (AOCT + NT - 1) / NT : 1;
-  constexpr int AEe = APL ? 1 : AE;
+  constexpr int AOCT = SPLIT * BM * (BK / 8);  // same for A
+  constexpr int AV = (AOCT + NT - 1) / NT;
   extern __shared__ __align__(16) char smem[];
-  __bf16* As = reinterpret_cast<__bf16*>(smem);       // [SPLIT][BM][LK]
-  __bf16* Bs = As + SPLIT * BM * LK;                   // [SPLIT][BN][LK]
+  __bf16* As = reinterpret_cast<__bf16*>(smem);  // [SPLIT][BM][LK]
+  __bf16* Bs = As + SPLIT * BM * LK;              // [SPLIT][BN][LK]
   if (g.run_if && *g.run_if == 0) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -169,45 +178,26 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
   kchunk = (kchunk + BK - 1) / BK * BK;
   const int kbeg = kpart * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
-  const int64_t plane = (int64_t)g.N * Kp;
+  const int64_t bplane = (int64_t)g.N * Kp;
 
-  const bool a_mfast = contiguous(g.am) || !contiguous(g.ak);
-  int a_ml[AEe], a_kl[AEe];
-  int64_t a_moff[AEe];
-  bool a_mok[AEe];
-  if (!APL) {
+  // octet o -> (piece s, row, k-octet q); octets past the tile (o >= AOCT)
+  // re-read the last one and are not stored
+  int pa[AV], sa_off[AV];
 #pragma unroll
-    for (int i = 0; i < AEe; ++i) {
-      const int e = tid + i * NT;
-      if (a_mfast) { a_ml[i] = e % BM; a_kl[i] = e / BM; }
-      else         { a_kl[i] = e % BK; a_ml[i] = e / BK; }
-      const int m = m0 + a_ml[i];
-      a_mok[i] = m < g.M;
-      a_moff[i] = a_mok[i] ? imap(g.am, m) : 0;
-    }
+  for (int i = 0; i < AV; ++i) {
+    const int o = min(tid + i * NT, AOCT - 1);
+    const int q = o % (BK / 8), ml = (o / (BK / 8)) % BM, s = o / (BK / 8 * BM);
+    pa[i] = s * (int)aplane + min(m0 + ml, g.M - 1) * Kp + 8 * q;
+    sa_off[i] = (s * BM + ml) * LK + 8 * q;
   }
-  int pa_s[AV], pa_ml[AV], pa_q[AV];
-  bool pa_ok[AV];
-  if (APL) {
-#pragma unroll
-    for (int i = 0; i < AV; ++i) {
-      const int o = tid + i * NT;
-      pa_q[i] = o % (BK / 8);
-      pa_ml[i] = (o / (BK / 8)) % BM;
-      pa_s[i] = o / (BK / 8 * BM);
-      pa_ok[i] = o < AOCT && m0 + pa_ml[i] < g.M;
-    }
-  }
-  // B octets: o -> (piece s, row nl, k-octet q)
-  int b_s[BV], b_nl[BV], b_q[BV];
-  bool b_ok[BV];
+  int pb[BV], sb_off[BV], b_k8[BV];
 #pragma unroll
   for (int i = 0; i < BV; ++i) {
-    const int o = tid + i * NT;
-    b_q[i] = o % (BK / 8);
-    b_nl[i] = (o / (BK / 8)) % BN;
-    b_s[i] = o / (BK / 8 * BN);
-    b_ok[i] = o < BOCT && n0 + b_nl[i] < g.N;
+    const int o = min(tid + i * NT, BOCT - 1);
+    const int q = o % (BK / 8), nl = (o / (BK / 8)) % BN, s = o / (BK / 8 * BN);
+    pb[i] = s * (int)bplane + min(n0 + nl, g.N - 1) * Kp + 8 * q;
+    sb_off[i] = (s * BN + nl) * LK + 8 * q;
+    b_k8[i] = 8 * q;
   }
 
   f4 acc[WM][WN];
@@ -216,57 +206,27 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
 #pragma unroll
     for (int j = 0; j < WN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  float ra[AEe];
-  u4 rpa[AV];
-  u4 rb[BV];
-  auto load_tile = [&](int kt) {
-    if (APL) {
+  auto load_a = [&](u4 (&ra)[AV], int kt) {
 #pragma unroll
-      for (int i = 0; i < AV; ++i)
-        rpa[i] = pa_ok[i] ? *reinterpret_cast<const u4*>(
-                                &Ap[pa_s[i] * aplane + (int64_t)(m0 + pa_ml[i]) * Kp + kt + 8 * pa_q[i]])
-                          : u4{0u, 0u, 0u, 0u};
-    } else {
-#pragma unroll
-      for (int i = 0; i < AEe; ++i) {
-        const int k = kt + a_kl[i];
-        ra[i] = (a_mok[i] && k < kend) ? (float)A[a_moff[i] + imap(g.ak, k)] : 0.f;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < BV; ++i) {
-      // the planes are zero beyond K (kt + 32 <= Kp always); k >= kend inside
-      // K belongs to another split-K part and is masked when stored to LDS
-      rb[i] = b_ok[i] ? *reinterpret_cast<const u4*>(
-                            &Bp[b_s[i] * plane + (int64_t)(n0 + b_nl[i]) * Kp + kt + 8 * b_q[i]])
-                      : u4{0u, 0u, 0u, 0u};
-    }
+    for (int i = 0; i < AV; ++i) ra[i] = *reinterpret_cast<const u4*>(Ap + (pa[i] + kt));
   };
-
-  if (kbeg < kend) load_tile(kbeg);
-  for (int kt = kbeg; kt < kend; kt += BK) {
-    __syncthreads();
-    if (APL) {
-      // A and B planes are zero beyond K; inside K, k >= kend is zeroed in B
-      // only (one zero factor per product suffices)
+  auto load_b = [&](u4 (&rb)[BV], int kt) {
 #pragma unroll
-      for (int i = 0; i < AV; ++i)
-        if (tid + i * NT < AOCT)
-          *reinterpret_cast<u4*>(&As[(pa_s[i] * BM + pa_ml[i]) * LK + 8 * pa_q[i]]) = rpa[i];
-    } else {
+    for (int i = 0; i < BV; ++i) rb[i] = *reinterpret_cast<const u4*>(Bp + (pb[i] + kt));
+  };
+  // registers -> LDS; inside K, k >= kend belongs to another split-K part and
+  // is zeroed in B only (one zero factor per product suffices; both planes
+  // are zero beyond K)
+  auto stage = [&](const u4 (&ra)[AV], const u4 (&rb)[BV], int kt) {
 #pragma unroll
-      for (int i = 0; i < AEe; ++i) {
-        __bf16 pc[SPLIT];
-        split<SPLIT>(ra[i], pc);
-#pragma unroll
-        for (int s = 0; s < SPLIT; ++s) As[(s * BM + a_ml[i]) * LK + a_kl[i]] = pc[s];
-      }
-    }
+    for (int i = 0; i < AV; ++i)
+      if (AOCT % NT == 0 || tid + i * NT < AOCT)
+        *reinterpret_cast<u4*>(&As[sa_off[i]]) = ra[i];
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
-      if (tid + i * NT < BOCT) {
+      if (BOCT % NT == 0 || tid + i * NT < BOCT) {
         u4 v = rb[i];
-        const int k0 = kt + 8 * b_q[i];
+        const int k0 = kt + b_k8[i];
         if (k0 + 8 > kend) {
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
@@ -275,36 +235,54 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
             else if (ka + 1 >= kend) v[t] &= 0x0000FFFFu;
           }
         }
-        *reinterpret_cast<u4*>(&Bs[(b_s[i] * BN + b_nl[i]) * LK + 8 * b_q[i]]) = v;
+        *reinterpret_cast<u4*>(&Bs[sb_off[i]]) = v;
       }
     }
-    __syncthreads();
-    if (kt + BK < kend) load_tile(kt + BK);
+  };
+  // smallest piece products first; the A fragments of every piece stay in
+  // registers, B fragments are re-read from LDS per product (fewer live
+  // registers, more resident waves)
+  auto compute = [&]() {
     const int k0 = 8 * (lane >> 4);
-    bf8 a[SPLIT][WM], b[SPLIT][WN];
+    bf8 a[SPLIT][WM];
 #pragma unroll
-    for (int s = 0; s < SPLIT; ++s) {
+    for (int s = 0; s < SPLIT; ++s)
 #pragma unroll
       for (int i = 0; i < WM; ++i)
         a[s][i] = *reinterpret_cast<const bf8*>(
             &As[(s * BM + wm * 16 * WM + i * 16 + (lane & 15)) * LK + k0]);
 #pragma unroll
-      for (int j = 0; j < WN; ++j)
-        b[s][j] = *reinterpret_cast<const bf8*>(
-            &Bs[(s * BN + wn * 16 * WN + j * 16 + (lane & 15)) * LK + k0]);
-    }
-    // smallest piece products first
-#pragma unroll
     for (int ord = SPLIT - 1; ord >= 0; --ord)
 #pragma unroll
       for (int sa = 0; sa <= ord; ++sa) {
         const int sb = ord - sa;
+        bf8 b[WN];
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+          b[j] = *reinterpret_cast<const bf8*>(
+              &Bs[(sb * BN + wn * 16 * WN + j * 16 + (lane & 15)) * LK + k0]);
 #pragma unroll
         for (int i = 0; i < WM; ++i)
 #pragma unroll
           for (int j = 0; j < WN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[sb][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[j], acc[i][j], 0, 0, 0);
       }
+  };
+
+  u4 ra0[AV], rb[BV];
+  if (kbeg < kend) {
+    load_b(rb, kbeg);
+    load_a(ra0, kbeg);
+  }
+  for (int kt = kbeg; kt < kend; kt += BK) {
+    __syncthreads();
+    stage(ra0, rb, kt);
+    __syncthreads();
+    if (kt + BK < kend) {
+      load_b(rb, kt + BK);
+      load_a(ra0, kt + BK);
+    }
+    compute();
   }
 
 #pragma unroll
@@ -316,28 +294,14 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(GemmDesc g, const double
         const int m = m0 + wm * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
         const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
         if (m >= g.M || n >= g.N) continue;
-        const double v = (double)acc[i][j][r];
         if (ksplit == 1) {
+          const double v = (double)acc[i][j][r];
           const int64_t off = imap(g.cm, m) + imap(g.cn, n);
           C[off] = (g.beta != 0.0) ? g.alpha * v + g.beta * C[off] : g.alpha * v;
         } else {
-          work[((int64_t)kpart * g.M + m) * g.N + n] = v;
+          work[((int64_t)kpart * g.M + m) * g.N + n] = acc[i][j][r];
         }
       }
-}
-
-__global__ void splitk_reduce_split(GemmDesc g, double* __restrict__ C, int ksplit,
-                                    const double* __restrict__ work) {
-  if (g.run_if && *g.run_if == 0) return;
-  const int64_t mn = (int64_t)g.M * g.N;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int p = 0; p < ksplit; ++p) s += work[(int64_t)p * mn + e];
-    const int m = (int)(e / g.N), n = (int)(e % g.N);
-    const int64_t off = imap(g.cm, m) + imap(g.cn, n);
-    C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
-  }
 }
 
 // General products (operands FP64 through their index maps, split in
@@ -490,8 +454,17 @@ __global__ void splitk_reduce_f32(GemmDesc g, double* __restrict__ C, int ksplit
   const int64_t mn = (int64_t)g.M * g.N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
        e += (int64_t)gridDim.x * blockDim.x) {
+    // eight partial loads in flight per step; the sum keeps the part order
     double s = 0.0;
-    for (int p = 0; p < ksplit; ++p) s += (double)work[(int64_t)p * mn + e];
+    int p = 0;
+    for (; p + 8 <= ksplit; p += 8) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = work[(int64_t)(p + t) * mn + e];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) s += (double)v[t];
+    }
+    for (; p < ksplit; ++p) s += (double)work[(int64_t)p * mn + e];
     const int m = (int)(e / g.N), n = (int)(e % g.N);
     const int64_t off = imap(g.cm, m) + imap(g.cn, n);
     C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
@@ -563,66 +536,108 @@ Tile choose_tile(const GemmDesc& g) {
   return Tile{2, 2};
 }
 
-int choose_ksplit(const GemmDesc& g) {
+// Workgroups of one planar kernel instance resident per CU (registers and
+// LDS), from the runtime's occupancy calculator; 1 if it cannot say.
+template <int WM, int WN, int SPLIT>
+int planar_wgs_per_cu() {
+  static int n = 0;
+  if (n == 0) {
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
+    const void* f = (const void*)gemm_split_kernel<WM, WN, SPLIT>;
+    ACETN_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, f, NT, lds) != hipSuccess || v < 1) v = 1;
+    n = v;
+  }
+  return n;
+}
+
+template <int SPLIT>
+int planar_wgs_per_cu(const Tile& t) {
+  if (t.wm == 2 && t.wn == 5) return planar_wgs_per_cu<2, 5, SPLIT>();
+  if (t.wm == 1 && t.wn == 5) return planar_wgs_per_cu<1, 5, SPLIT>();
+  if (t.wm == 4 && t.wn == 4) return planar_wgs_per_cu<4, 4, SPLIT>();
+  return planar_wgs_per_cu<2, 2, SPLIT>();
+}
+
+// Split-K count: the largest that keeps tiles x parts within one round of
+// resident workgroups on the 256 CUs (a second, partly filled round costs as
+// much as a full one), each part at least 4 k-tiles long.  per_cu = 0 gives
+// the upper bound used to size the scratch (LDS limit only).
+int choose_ksplit(const GemmDesc& g, int split, int per_cu) {
   const Tile t = choose_tile(g);
   const int BM = 32 * t.wm, BN = 32 * t.wn;
   const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
-  if (tiles >= 512 || g.K < 8 * BK * 4) return 1;
-  int ks = (int)((1024 + tiles - 1) / tiles);
+  if (per_cu <= 0) {
+    const size_t lds = sizeof(__bf16) * (size_t)split * (BM + BN) * (BK + KPAD);
+    per_cu = std::max(1, (int)(160 * 1024 / lds));
+  }
+  const int64_t cap = (int64_t)kCUs * per_cu;
+  if (2 * tiles > cap || g.K < 8 * BK) return 1;
+  int ks = (int)(cap / tiles);
   ks = std::min(ks, g.K / (BK * 4));
   ks = std::min(ks, 64);
   return std::max(ks, 1);
 }
+int choose_ksplit_run(const GemmDesc& g, int split) {
+  const Tile t = choose_tile(g);
+  return choose_ksplit(g, split, split == 2 ? planar_wgs_per_cu<2>(t) : planar_wgs_per_cu<3>(t));
+}
+int ksplit_bound(const GemmDesc& g) {
+  return std::max(choose_ksplit(g, 2, 0), choose_ksplit(g, 3, 0));
+}
 
 int padded_k(int K) { return (std::max(K, 1) + BK - 1) / BK * BK; }
 
-// scratch (in doubles): split-K partials, then the B planes
+// scratch (in doubles): FP32 split-K partials, then the B planes
 // (rounded to 32 doubles so that the planes start 256-byte aligned)
 size_t partial_elems(const GemmDesc& g, int ks) {
-  return ks > 1 ? ((size_t)ks * g.M * g.N + 31) / 32 * 32 : 0;
+  return ks > 1 ? (((size_t)ks * g.M * g.N + 1) / 2 + 31) / 32 * 32 : 0;
 }
 size_t plane_elems(const GemmDesc& g, int split) {
   const size_t bytes = (size_t)split * g.N * padded_k(g.K) * sizeof(__bf16);
   return (bytes + 255) / 256 * 256 / sizeof(double);
 }
 
-template <int WM, int WN, int SPLIT, bool APL>
-void launch(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, const __bf16* Bp,
-            int Kp, double* C, int ks, double* work, hipStream_t s) {
+template <int WM, int WN, int SPLIT>
+void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* Bp, int Kp,
+            double* C, int ks, double* work, hipStream_t s) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
   const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
-  static bool attr = false;
-  if (!attr) {
-    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)gemm_split_kernel<WM, WN, SPLIT, APL>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr = true;
-  }
+  planar_wgs_per_cu<WM, WN, SPLIT>();  // sets the dynamic-LDS attribute once
   dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, ks);
   ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_split: grid too large");
+  float* wf = reinterpret_cast<float*>(work);
   hipEvent_t e0 = kernel_timing_begin(s);
-  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT, APL>), grid, dim3(NT), lds, s, g, A, Ap,
-                     aplane, Bp, Kp, C, ks, work);
+  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT>), grid, dim3(NT), lds, s, g, Ap, aplane, Bp,
+                     Kp, C, ks, wf);
   ACETN_HIP_CHECK(hipGetLastError());
   kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
   if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     const int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
-    hipLaunchKernelGGL(splitk_reduce_split, dim3(blocks), dim3(256), 0, s, g, C, ks, work);
+    hipLaunchKernelGGL(splitk_reduce_f32, dim3(blocks), dim3(256), 0, s, g, C, ks, wf);
     ACETN_HIP_CHECK(hipGetLastError());
   }
 }
 
-template <int SPLIT, bool APL>
-void run(const GemmDesc& g, const double* A, const __bf16* Ap, int64_t aplane, const double* B,
-         double* C, int ks, double* work, hipStream_t s) {
+// C = A B with A given as planes; B (FP64 through its index maps) is split
+// into planes after the partials in the scratch.
+template <int SPLIT>
+void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, double* C, int ks,
+         double* work, hipStream_t s) {
   const int Kp = padded_k(g.K);
+  ACETN_REQUIRE((int64_t)SPLIT * aplane < (int64_t(1) << 31) &&
+                    (int64_t)SPLIT * g.N * Kp < (int64_t(1) << 31),
+                "gemm_split: operand planes exceed 2^31 elements");
   __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, ks));
   split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
   const Tile t = choose_tile(g);
-  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
-  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
-  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
-  else launch<2, 2, SPLIT, APL>(g, A, Ap, aplane, planes, Kp, C, ks, work, s);
+  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
+  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
+  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
+  else launch<2, 2, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
 }
 }  // namespace
 
@@ -632,7 +647,7 @@ constexpr double kPlanarFlops = 1e8;
 
 size_t gemm_split_work_elems(const GemmDesc& g, int count) {
   ACETN_REQUIRE(count == 1, "gemm_split: batched products are not supported");
-  return std::max(partial_elems(g, choose_ksplit(g)) + plane_elems(g, 3) +
+  return std::max(partial_elems(g, ksplit_bound(g)) + plane_elems(g, 3) +
                       split_planes_elems(g.M, g.K),
                   general_partial_elems(g, general_ksplit(g)));
 }
@@ -647,7 +662,7 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
   ACETN_REQUIRE(g.K > 0 || g.beta == 0.0, "gemm_split: K == 0 with beta != 0 unsupported");
   if (2.0 * g.M * g.N * (double)g.K >= kPlanarFlops) {
     const size_t apl = split_planes_elems(g.M, g.K);
-    int ks = choose_ksplit(g);
+    int ks = choose_ksplit_run(g, split);
     if (work_elems < partial_elems(g, ks) + plane_elems(g, 3) + apl) ks = 1;
     ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, ks) + plane_elems(g, 3) + apl,
                   "gemm_split: scratch too small");
@@ -657,8 +672,8 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
     // planes[s][m][k] of A(m, k) = A[am(m) + ak(k)]: the B-plane split with the maps swapped
     if (split == 2) split_into_planes<2>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s);
     else split_into_planes<3>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s);
-    if (split == 2) run<2, true>(g, nullptr, Ap, total, b.B[0], b.C[0], ks, work, s);
-    else run<3, true>(g, nullptr, Ap, total, b.B[0], b.C[0], ks, work, s);
+    if (split == 2) run<2>(g, Ap, total, b.B[0], b.C[0], ks, work, s);
+    else run<3>(g, Ap, total, b.B[0], b.C[0], ks, work, s);
     return;
   }
   int ks = general_ksplit(g);
@@ -675,12 +690,12 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
   if (g.M <= 0 || g.N <= 0) return;
   ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, 1) + plane_elems(g, split),
                 "gemm_split: scratch too small");
-  int ks = choose_ksplit(g);
+  int ks = choose_ksplit_run(g, split);
   if (work_elems < partial_elems(g, ks) + plane_elems(g, split)) ks = 1;
   const int64_t aplane = (int64_t)A.rows * A.kp;
   const __bf16* Ap = reinterpret_cast<const __bf16*>(A.p);
-  if (split == 2) run<2, true>(g, nullptr, Ap, aplane, B, C, ks, work, s);
-  else run<3, true>(g, nullptr, Ap, aplane, B, C, ks, work, s);
+  if (split == 2) run<2>(g, Ap, aplane, B, C, ks, work, s);
+  else run<3>(g, Ap, aplane, B, C, ks, work, s);
 }
 
 size_t split_planes_elems(int rows, int K) {

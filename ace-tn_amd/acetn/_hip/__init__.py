@@ -230,6 +230,9 @@ def serialize(flag):
     _serial = bool(flag)
 
 
+N_SLOTS = 4  # concurrent projector streams (one per hardware queue)
+
+
 def slot_stream(slot, device):
     """Default-priority streams, one per concurrent slot.  Slot 0 runs on the
     caller's current stream: the runtime spreads streams over a small fixed
@@ -391,7 +394,7 @@ class ProjectorJob:
     order; system 'half-system' | 'full-system', svd 'rsvd' | 'full-rank'."""
 
     def __init__(self, slot, sites, k, chi, oversampling, niter, cutoff, seed, stream_id,
-                 precision="fp64", system="half-system", svd="rsvd"):
+                 precision="fp64", system="half-system", svd="rsvd", ws_key=None):
         made = [make_site(*st) for st in sites]
         self.sv = [m[0] for m in made]
         self.keep = [m[1] for m in made]
@@ -406,7 +409,9 @@ class ProjectorJob:
         self.stream = slot_stream(slot, self.dev)
         nb = lib().acetn_hip_projectors_workspace(self.arr, k, chi, oversampling, self.system,
                                                   self.svd)
-        self.ws = slot_workspace(slot, nb, self.dev)
+        # pairs queued behind each other on one slot keep their own scratch
+        # until their second phase has run
+        self.ws = slot_workspace(slot if ws_key is None else ("pair", ws_key), nb, self.dev)
         self.cd_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
         nspec = max(chi + oversampling, 512) if self.svd else chi + oversampling
         self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
@@ -431,9 +436,11 @@ class ProjectorJob:
         return P1, P2
 
 
-def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None, precision="fp64"):
+def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None, precision="fp64",
+                         ws_key=None):
     """Returns (C_a, C_b, E_o): the new C[(3+k)%4], C[k], E[(3+k)%4] of s2.
-    ``slot`` selects a per-slot workspace for concurrent calls."""
+    ``slot`` (or ``ws_key``, when several calls share a slot) selects a
+    workspace of its own for concurrent calls."""
     s1, keep = make_site(*site1)
     dev = keep[0].device
     D = s1.D
@@ -447,7 +454,8 @@ def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None, precision="fp6
     nb = lib().acetn_hip_renormalize_workspace(ctypes.byref(s1), k, cDi, cDj)
     # own per-slot scratch: a renormalisation may run while other slots'
     # projector pairs still use theirs
-    ws = workspace(nb, dev) if slot is None else slot_workspace(("renormalize", slot), nb, dev)
+    key = slot if ws_key is None else ws_key
+    ws = workspace(nb, dev) if key is None else slot_workspace(("renormalize", key), nb, dev)
     check(lib().acetn_hip_renormalize_boundary(ctypes.byref(s1), k, ptr(P1i), ptr(P2i), cDi,
                                                ptr(P1j), ptr(P2j), cDj, precision_code(precision),
                                                ptr(Ca), ptr(Cb), ptr(Eo),

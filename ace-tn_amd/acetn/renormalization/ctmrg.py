@@ -34,6 +34,26 @@ def ctmrg_iteration(mover, ipeps):
         mover.stage(ipeps, [((ny - yi + 1) % ny, 1), (yi, 3)])
 
 
+def stage_moves(ipeps):
+    """The stages of one iteration in ctmrg_iteration's order."""
+    nx, ny = ipeps.nx, ipeps.ny
+    return ([[(xi, 0), ((nx - xi + 1) % nx, 2)] for xi in range(nx)] +
+            [[((ny - yi + 1) % ny, 1), (yi, 3)] for yi in range(ny)])
+
+
+def ctmrg_iteration_many(pairs):
+    """One CTMRG iteration of each of several independent networks (a
+    parameter sweep: instances of uneven D and chi), ``pairs`` a list of
+    (mover, ipeps).  Stage s of every network runs in one round
+    (run_stages), so one instance's single-workgroup eigensolves overlap the
+    others' matrix-core work; each network sees exactly the operations of
+    ctmrg_iteration, in the same order, so its result is the same."""
+    from .directional_mover import run_stages
+    plans = [(m, net, stage_moves(net)) for m, net in pairs]
+    for s in range(max(len(st) for _, _, st in plans)):
+        run_stages([(m, net, st[s]) for m, net, st in plans if s < len(st)])
+
+
 def ctmrg(ipeps, config, mover=None):
     """Runs ``config.steps`` CTMRG iterations; with ``config.conv_tol`` set,
     stops early once the normalised corner spectra change by less than it."""

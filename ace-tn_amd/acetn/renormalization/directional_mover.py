@@ -43,13 +43,11 @@ class DirectionalMover:
         self._stage(ipeps, moves)
 
     def _stage(self, ipeps, moves):
-        """Projector pairs of every plaquette of the moves, then the
-        renormalisations.  A renormalisation starts as soon as both of its
-        projector pairs exist (its slot stream waits on their completion
-        events), so the first move's renormalisations overlap the later
-        pairs' projector work; the new boundary tensors are written back
-        after everything has finished."""
-        import torch
+        run_stages([(self, ipeps, moves)])
+
+    def _plan(self, ipeps, moves):
+        """Projector pairs (plaquette, k) of the moves and the renormalisations
+        (s1, s2, i, j, k) that consume pairs i and j."""
         jobs, targets = [], []
         for line, k in moves:
             n_other = ipeps.ny if k in (0, 2) else ipeps.nx
@@ -60,27 +58,7 @@ class DirectionalMover:
             for idx in range(n_other):
                 s1, s2, i, j = self.targets(ipeps.nx, ipeps.ny, line, k, idx)
                 targets.append((s1, s2, base + i, base + j, k))
-        calc = self.projector_calculator
-        started = calc.start_many(ipeps, jobs)
-        proj, done = [None] * len(jobs), [None] * len(jobs)
-        results = [None] * len(targets)
-        for idx, pr, ev in calc.finish_each(ipeps, started):
-            proj[idx], done[idx] = pr, ev
-            for slot, (s1, s2, i, j, k) in enumerate(targets):
-                if results[slot] is not None or proj[i] is None or proj[j] is None:
-                    continue
-                stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
-                stream.wait_event(done[i])
-                stream.wait_event(done[j])
-                with torch.cuda.stream(stream):
-                    results[slot] = self._renormalize(ipeps, proj[i], proj[j], s1, k, slot,
-                                                      calc.precision)
-        torch.cuda.synchronize()
-        for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(targets, results):
-            dst = ipeps[s2]
-            dst['C'][(3 + k) % 4] = Ca
-            dst['C'][k] = Cb
-            dst['E'][(3 + k) % 4] = Eo
+        return jobs, targets
 
     def _move(self, ipeps, line, k):
         self.stage(ipeps, [(line, k)])
@@ -134,8 +112,66 @@ class DirectionalMover:
         dst['E'][(3 + k) % 4] = Eo
 
     @staticmethod
-    def _renormalize(ipeps, pi, pj, s1, k, slot=None, precision="fp64"):
+    def _renormalize(ipeps, pi, pj, s1, k, slot=None, precision="fp64", ws_key=None):
         st = ipeps[s1]
         flat = lambda P: P.reshape(-1, P.shape[-1])
         return _hip.renormalize_boundary((st['A'], st['C'], st['E']), k, flat(pi[0]), flat(pi[1]),
-                                         flat(pj[0]), flat(pj[1]), slot=slot, precision=precision)
+                                         flat(pj[0]), flat(pj[1]), slot=slot, precision=precision,
+                                         ws_key=ws_key)
+
+
+def run_stages(items):
+    """One stage of each of several independent networks, run as one round.
+
+    ``items`` is a list of (mover, ipeps, moves).  Every projector pair of
+    every item is started first (a single network's pair i on slot i, the
+    pairs of a multi-network round on slot g % N_SLOTS in round order g, each
+    with a workspace of its own); the host then reads each pair's truncation
+    count as soon as that pair has produced it, enqueues its projector
+    products, and starts a renormalisation as soon as both of its pairs are
+    finished (its slot stream waits on their completion events).  One
+    network's Jacobi eigensolves thereby overlap the other pairs' matrix-core
+    work.  The new boundary tensors are written back after everything has
+    finished.  With full-system projectors the two moves of a stage depend on
+    each other, so each network then runs its stage on its own."""
+    import torch
+    if any(m.projector_calculator.projectors != "half-system" and len(mv) > 1
+           for m, _, mv in items):
+        for m, net, mv in items:
+            m.stage(net, mv)
+        return
+    multi = len(items) > 1
+    plans, base = [], 0
+    for mover, ipeps, moves in items:
+        jobs, targets = mover._plan(ipeps, moves)
+        calc = mover.projector_calculator
+        started = calc.start_many(ipeps, jobs, base=base if multi else None)
+        plans.append({"mover": mover, "ipeps": ipeps, "targets": targets, "started": started,
+                      "base": base, "proj": [None] * len(jobs), "done": [None] * len(jobs),
+                      "results": [None] * len(targets)})
+        base += len(jobs)
+    for pl in plans:
+        mover, ipeps = pl["mover"], pl["ipeps"]
+        calc = mover.projector_calculator
+        proj, done, results = pl["proj"], pl["done"], pl["results"]
+        for idx, pr, ev in calc.finish_each(ipeps, pl["started"]):
+            proj[idx], done[idx] = pr, ev
+            for t, (s1, s2, i, j, k) in enumerate(pl["targets"]):
+                if results[t] is not None or proj[i] is None or proj[j] is None:
+                    continue
+                g = pl["base"] + t
+                slot = g % _hip.N_SLOTS if multi else t
+                stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
+                stream.wait_event(done[i])
+                stream.wait_event(done[j])
+                with torch.cuda.stream(stream):
+                    results[t] = mover._renormalize(ipeps, proj[i], proj[j], s1, k, slot,
+                                                    calc.precision,
+                                                    ws_key=g if multi else None)
+    torch.cuda.synchronize()
+    for pl in plans:
+        for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(pl["targets"], pl["results"]):
+            dst = pl["ipeps"][s2]
+            dst['C'][(3 + k) % 4] = Ca
+            dst['C'][k] = Cb
+            dst['E'][(3 + k) % 4] = Eo

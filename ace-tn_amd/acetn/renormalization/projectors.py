@@ -65,16 +65,21 @@ class ProjectorCalculator:
         torch.cuda.synchronize()
         return [pr for _, pr, _ in out]
 
-    def start_many(self, ipeps, jobs, system=None):
-        """Enqueue every projector pair up to its truncation count (one
-        stream per pair); returns the started jobs in ``jobs`` order."""
+    def start_many(self, ipeps, jobs, system=None, base=None):
+        """Enqueue every projector pair up to its truncation count; returns
+        the started jobs in ``jobs`` order.  Pair i runs on slot i (one
+        stream per pair); with ``base`` set (several networks in one round),
+        pair i is round-wide pair base + i on slot (base + i) % N_SLOTS with
+        a workspace of its own."""
         system = system or ("full-system" if self.calculate == self.calculate_full_system
                             else "half-system")
         if self.svd_type not in ("rsvd", "full-rank"):
             raise ValueError(f"Invalid svd_type: {self.svd_type}")
         chi = ipeps.dims['chi']
         started = []
-        for slot, (sites, k) in enumerate(jobs):
+        for i, (sites, k) in enumerate(jobs):
+            slot = i if base is None else (base + i) % _hip.N_SLOTS
+            ws_key = None if base is None else base + i
             sites = [tuple(s) for s in sites]
             for s in sites:
                 ipeps[s]  # raises ValueError for an unknown site
@@ -83,7 +88,7 @@ class ProjectorCalculator:
                                              chi, self.rsvd_oversampling, self.rsvd_niter,
                                              self.svd_cutoff, self.sketch.seed, stream_id,
                                              precision=self.precision, system=system,
-                                             svd=self.svd_type))
+                                             svd=self.svd_type, ws_key=ws_key))
         return started
 
     def finish_each(self, ipeps, started):

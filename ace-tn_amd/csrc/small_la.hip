@@ -348,9 +348,12 @@ __global__ __launch_bounds__(CB_NT) void chol_rinv_blk_kernel(const double* __re
   }
   __syncthreads();
 
+  // Look-ahead: wave 0 brings the next diagonal tile up to date first and
+  // factors it while waves 1.. update the rest of the trailing matrix, so a
+  // tile column costs one panel pass and one trailing pass (two barriers).
+  if (wave == 0) cb_factor_diag(T + cb_tile(0, 0), xbuf, lane, thresh, dead);
+  __syncthreads();
   for (int k = 0; k < Q; ++k) {
-    if (wave == 0) cb_factor_diag(T + cb_tile(k, k), xbuf, lane, thresh, dead + 16 * k);
-    __syncthreads();
     for (int jj = k + 1 + wave; jj < Q; jj += CB_WAVES) {
       cb_d4 acc = {0.0, 0.0, 0.0, 0.0};
       cb_mma<true>(acc, T + cb_tile(k, k), T + cb_tile(k, jj), lane, 1.0);
@@ -358,15 +361,26 @@ __global__ __launch_bounds__(CB_NT) void chol_rinv_blk_kernel(const double* __re
       cb_store_acc(T + cb_tile(k, jj), lane, acc);
     }
     __syncthreads();
+    if (k + 1 == Q) break;
     const int nrem = Q - k - 1;
     const int ntr = nrem * (nrem + 1) / 2;
-    for (int u = wave; u < ntr; u += CB_WAVES) {
-      int Jp = 0;
-      while ((Jp + 1) * (Jp + 2) / 2 <= u) ++Jp;
-      const int I = k + 1 + (u - Jp * (Jp + 1) / 2), J = k + 1 + Jp;
-      cb_d4 acc = cb_load_acc(T + cb_tile(I, J), lane);
-      cb_mma<true>(acc, T + cb_tile(k, I), T + cb_tile(k, J), lane, -1.0);
-      cb_store_acc(T + cb_tile(I, J), lane, acc);
+    if (wave == 0) {
+      // trailing tile u = 0 is the next diagonal tile (k + 1, k + 1)
+      cb_d4 acc = cb_load_acc(T + cb_tile(k + 1, k + 1), lane);
+      cb_mma<true>(acc, T + cb_tile(k, k + 1), T + cb_tile(k, k + 1), lane, -1.0);
+      cb_wave_sync();
+      cb_store_acc(T + cb_tile(k + 1, k + 1), lane, acc);
+      cb_wave_sync();
+      cb_factor_diag(T + cb_tile(k + 1, k + 1), xbuf, lane, thresh, dead + 16 * (k + 1));
+    } else {
+      for (int u = wave; u < ntr; u += CB_WAVES - 1) {
+        int Jp = 0;
+        while ((Jp + 1) * (Jp + 2) / 2 <= u) ++Jp;
+        const int I = k + 1 + (u - Jp * (Jp + 1) / 2), J = k + 1 + Jp;
+        cb_d4 acc = cb_load_acc(T + cb_tile(I, J), lane);
+        cb_mma<true>(acc, T + cb_tile(k, I), T + cb_tile(k, J), lane, -1.0);
+        cb_store_acc(T + cb_tile(I, J), lane, acc);
+      }
     }
     __syncthreads();
   }

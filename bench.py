@@ -252,6 +252,83 @@ def run_als(args):
     return res
 
 
+def run_sweep(args):
+    """Parameter sweep of independent instances: 12 transverse-field Ising
+    iPEPS (2x2 cell, d=2, J=1, h uniform in [2.5, 3.5]), D drawn from
+    {3, 4, 5, 6}, chi = 4 D^2, site tensors seeded N(0,1) + 0.5 on the
+    spin-up component, mixed-precision contractions and FP64 projectors.
+    CTMRG only (the field h enters the energy, not the boundary), so h is
+    reported with each instance.  The instances run together
+    (ctmrg_iteration_many) and, for comparison, one after another with the
+    single-network driver; both must leave identical tensors."""
+    import torch
+    from acetn.renormalization.ctmrg import ctmrg_iteration, ctmrg_iteration_many
+    from acetn.renormalization.directional_mover import DirectionalMover
+    g = torch.Generator().manual_seed(args.seed)
+    Ds = [int(x) for x in torch.randint(3, 7, (12,), generator=g)]
+    hs = [float(x) for x in 2.5 + torch.rand(12, generator=g, dtype=torch.float64)]
+    insts = [(D, 4 * D * D, args.seed * 100 + i) for i, D in enumerate(Ds)]
+
+    def nets():
+        out = []
+        for D, chi, seed in insts:
+            net = DeviceNetwork(random_state(D, 2, seed, 0.5), D, chi, seed=seed)
+            out.append((DirectionalMover(_Cfg("mixed"), net.sketch), net))
+        return out
+
+    steps, warm = args.sweep_steps, 2
+    # together
+    pairs = nets()
+    for _ in range(warm):
+        ctmrg_iteration_many(pairs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctmrg_iteration_many(pairs)
+    torch.cuda.synchronize()
+    dt_many = time.perf_counter() - t0
+    # one after another
+    alone = nets()
+    for m, net in alone:
+        for _ in range(warm):
+            ctmrg_iteration(m, net)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m, net in alone:
+        for _ in range(steps):
+            ctmrg_iteration(m, net)
+    torch.cuda.synchronize()
+    dt_seq = time.perf_counter() - t0
+    same = all(torch.equal(a[s][key][k], b[s][key][k])
+               for (_, a), (_, b) in zip(pairs, alone) for s in a.site_list
+               for key in ('C', 'E') for k in range(4))
+    flops = sum(ctmrg_flops(D, chi) for D, chi, _ in insts)
+    res = {"workload": "12 transverse-field Ising iPEPS, 2x2 cell, d=2, J=1, D drawn from "
+                       "{3,4,5,6}, chi=4D^2, mixed-precision contractions, FP64 projectors",
+           "instances": [{"D": D, "chi": chi, "h": h} for (D, chi, _), h in zip(insts, hs)],
+           "steps": steps, "warmup": warm,
+           "instance_iters_per_s_together": 12 * steps / dt_many,
+           "instance_iters_per_s_one_by_one": 12 * steps / dt_seq,
+           "speedup_together": dt_seq / dt_many,
+           "tflops_together": flops * steps / dt_many / 1e12,
+           "identical_tensors": same}
+    log(f"[sweep] {res}")
+    return res
+
+
+def ctmrg_flops(D, chi, d=2):
+    """Algorithmic FLOPs of one CTMRG iteration (2x2 cell, half-system rSVD,
+    q = chi + 2, 2 power iterations): 16 projector pairs of two quarter
+    tensors, 12 skinny range-finder products, projector products and the
+    renormalisation of C and E (the per-row counts of the hot-path table)."""
+    N, q = chi * D * D, chi + 2
+    quarter = 2 * chi ** 3 * D ** 2 + 2 * chi ** 3 * D ** 4 + 4 * chi ** 2 * D ** 6 * d
+    rsvd = 24 * N * N * q
+    proj = 4 * N * N * chi
+    renorm = 4 * chi ** 3 * D ** 4 + 4 * chi ** 2 * D ** 6 * d
+    return 16 * (2 * quarter + rsvd + proj) + 16 * renorm
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +342,9 @@ def main():
                     help="fast-full-update sweeps recording ALS problems (8 per sweep)")
     ap.add_argument("--als-epochs", type=int, default=60)
     ap.add_argument("--no-als", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the 12-instance parameter sweep")
+    ap.add_argument("--sweep-steps", type=int, default=3)
     ap.add_argument("--serial", action="store_true",
                     help="run the timed iterations with the projector pairs serialised")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
@@ -336,6 +416,8 @@ def main():
                      "avg_launch_ms": st["ms"] / st["launches"]},
         "kernels": {"mixed": mix["stats"], "fp64": ref["stats"]},
     }
+    if not args.no_sweep:
+        out["sweep"] = run_sweep(args)
     if not args.no_als:
         out["als"] = run_als(args)
     if not args.no_cpu_baseline:

@@ -209,7 +209,11 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
   }
   const int64_t nblk = (int64_t)g.n1 * g.n2;
   const int64_t ngroups = (nblk + S::NB - 1) / S::NB;
-  const int grid = (int)std::min<int64_t>(ngroups, 4096);
+  // one round of resident workgroups (LDS-limited, 256 CUs), each looping
+  // over its share of the groups with the next group's X prefetched: the A
+  // tensor is staged once per workgroup instead of once per group or two
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
+  const int grid = (int)std::min<int64_t>(ngroups, 256 * per_cu);
   const double d6 = (double)S::D2 * S::D2 * S::D2;
   const double flops = (double)nblk * (2.0 * g.d * d6 + 2.0 * (g.open_p ? g.d * g.d : g.d) * d6);
   hipEvent_t e0 = kernel_timing_begin(s);

@@ -98,6 +98,9 @@ __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; 
 // element's address is base + row*stride + k*stride: rows past the edge are
 // clamped to the last row (their results are never stored) and only the K
 // edge needs a predicate, which keeps the tile loads free of branches.
+// Otherwise the k part of the address (a mixed-radix map) is evaluated once
+// per k-tile and operand by 2*BK threads into a double-buffered LDS table
+// that every element load reads, instead of once per element.
 template <int WM, int WN, bool LIN>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, int ksplit,
                                                       double* __restrict__ work) {
@@ -106,6 +109,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
   constexpr int AE = BM * BK / NT, BE = BN * BK / NT;  // elements per thread
   __shared__ double As[2][BK][LA];
   __shared__ double Bs[2][BK][LB];
+  __shared__ int64_t koff[2][2][BK];  // [table][operand A / B][k within the tile]
   if (g.run_if && *g.run_if == 0) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -174,16 +178,24 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
       }
       return;
     }
+    const int tb = ((kt - kbeg) / BK) & 1;
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
       const int k = kt + a_kl[i];
-      ra[i] = (a_mok[i] && k < kend) ? A[a_moff[i] + imap(g.ak, k)] : 0.0;
+      ra[i] = (a_mok[i] && k < kend) ? A[a_moff[i] + koff[tb][0][a_kl[i]]] : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < BE; ++i) {
       const int k = kt + b_kl[i];
-      rb[i] = (b_nok[i] && k < kend) ? B[imap(g.bk, k) + b_noff[i]] : 0.0;
+      rb[i] = (b_nok[i] && k < kend) ? B[koff[tb][1][b_kl[i]] + b_noff[i]] : 0.0;
     }
+  };
+  // k-offset table of the k-tile at kt (general maps only)
+  auto fill_koff = [&](int kt) {
+    if (LIN || tid >= 2 * BK) return;
+    const int tb = ((kt - kbeg) / BK) & 1, op = tid / BK, j = tid % BK;
+    const int k = min(kt + j, kend - 1);
+    koff[tb][op][j] = op == 0 ? imap(g.ak, k) : imap(g.bk, k);
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
@@ -194,8 +206,13 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
 
   int buf = 0;
   if (kbeg < kend) {
+    if (!LIN) {
+      fill_koff(kbeg);
+      __syncthreads();
+    }
     load_tile(kbeg);
     store_tile(0);
+    if (kbeg + BK < kend) fill_koff(kbeg + BK);
   }
   __syncthreads();
   for (int kt = kbeg; kt < kend; kt += BK) {
@@ -216,6 +233,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, 
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (more) store_tile(buf ^ 1);
+    // the table of k-tile kt + 2 BK replaces that of kt, read before the
+    // previous barrier
+    if (kt + 2 * BK < kend) fill_koff(kt + 2 * BK);
     __syncthreads();
     buf ^= 1;
   }
@@ -267,6 +287,85 @@ __global__ void splitk_reduce_kernel(GemmDesc g, GemmBatch bt, int ksplit,
   }
 }
 
+// Small-output, long-K products (M, N <= 8, e.g. the d^2 x d^2 reduced
+// density matrix summed over the whole environment): a 16 x 16 matrix-core
+// tile would be almost all padding, so each lane walks its own k indices and
+// accumulates all M x N dot products in registers; each workgroup reduces
+// its lanes (shuffles, then LDS across waves) into one partial, and a second
+// kernel sums the partials in a fixed order (bitwise reproducible).
+constexpr int kDotMax = 8, kDotNT = 256, kDotWGs = 512, kDotMinK = 4096;
+
+bool dot_small_fits(const GemmDesc& g) {
+  return g.M <= kDotMax && g.N <= kDotMax && g.K >= kDotMinK;
+}
+int dot_small_wgs(const GemmDesc& g) {
+  return (int)std::min<int64_t>(kDotWGs, ((int64_t)g.K + kDotNT - 1) / kDotNT);
+}
+
+__global__ __launch_bounds__(kDotNT) void dot_small_kernel(GemmDesc g, GemmBatch bt,
+                                                           double* __restrict__ part) {
+  constexpr int MN = kDotMax * kDotMax;
+  __shared__ double red[kDotNT / 64][MN];
+  if (g.run_if && *g.run_if == 0) return;
+  const int prob = blockIdx.y;
+  const double* __restrict__ A = bt.A[prob];
+  const double* __restrict__ B = bt.B[prob];
+  const int M = g.M, N = g.N;
+  int64_t aoff[kDotMax], boff[kDotMax];
+#pragma unroll
+  for (int m = 0; m < kDotMax; ++m) aoff[m] = m < M ? imap(g.am, m) : 0;
+#pragma unroll
+  for (int n = 0; n < kDotMax; ++n) boff[n] = n < N ? imap(g.bn, n) : 0;
+  double acc[kDotMax][kDotMax];
+#pragma unroll
+  for (int m = 0; m < kDotMax; ++m)
+#pragma unroll
+    for (int n = 0; n < kDotMax; ++n) acc[m][n] = 0.0;
+  for (int k = blockIdx.x * kDotNT + threadIdx.x; k < g.K; k += gridDim.x * kDotNT) {
+    const int64_t ka = imap(g.ak, k), kb = imap(g.bk, k);
+    double a[kDotMax], b[kDotMax];
+#pragma unroll
+    for (int m = 0; m < kDotMax; ++m) a[m] = m < M ? A[aoff[m] + ka] : 0.0;
+#pragma unroll
+    for (int n = 0; n < kDotMax; ++n) b[n] = n < N ? B[kb + boff[n]] : 0.0;
+#pragma unroll
+    for (int m = 0; m < kDotMax; ++m)
+#pragma unroll
+      for (int n = 0; n < kDotMax; ++n) acc[m][n] = fma(a[m], b[n], acc[m][n]);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < kDotMax; ++m)
+#pragma unroll
+    for (int n = 0; n < kDotMax; ++n) {
+      double v = acc[m][n];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+      if (lane == 0) red[wave][m * kDotMax + n] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < MN) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kDotNT / 64; ++w) v += red[w][threadIdx.x];
+    part[((int64_t)prob * gridDim.x + blockIdx.x) * MN + threadIdx.x] = v;
+  }
+}
+
+__global__ void dot_small_reduce(GemmDesc g, GemmBatch bt, const double* __restrict__ part,
+                                 int nwg) {
+  constexpr int MN = kDotMax * kDotMax;
+  if (g.run_if && *g.run_if == 0) return;
+  const int prob = blockIdx.x, t = threadIdx.x;
+  if (t >= g.M * g.N) return;
+  const int m = t / g.N, n = t % g.N;
+  const double* p = part + (int64_t)prob * nwg * MN + m * kDotMax + n;
+  double s = 0.0;
+  for (int w = 0; w < nwg; ++w) s += p[(int64_t)w * MN];
+  double* C = bt.C[prob];
+  const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+  C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
+}
+
 struct Tile {
   int wm, wn;
 };
@@ -303,6 +402,7 @@ void launch_gemm(const GemmDesc& g, const GemmBatch& b, int ks, double* work, hi
 }  // namespace
 
 size_t gemm_f64_work_elems(const GemmDesc& g, int count) {
+  if (dot_small_fits(g)) return (size_t)dot_small_wgs(g) * count * kDotMax * kDotMax;
   const int ks = choose_ksplit(g, count);
   return ks > 1 ? (size_t)ks * count * g.M * g.N : 0;
 }
@@ -314,6 +414,17 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
   if (g.K <= 0) {
     // C = beta*C (or zero): handled by a K=1 product with zero contribution
     ACETN_REQUIRE(g.beta == 0.0, "gemm_f64: K == 0 with beta != 0 unsupported");
+  }
+  if (dot_small_fits(g) && work != nullptr &&
+      work_elems >= (size_t)dot_small_wgs(g) * b.count * kDotMax * kDotMax) {
+    const int nwg = dot_small_wgs(g);
+    hipEvent_t e0 = kernel_timing_begin(s);
+    hipLaunchKernelGGL(dot_small_kernel, dim3(nwg, b.count), dim3(kDotNT), 0, s, g, b, work);
+    ACETN_HIP_CHECK(hipGetLastError());
+    kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count);
+    hipLaunchKernelGGL(dot_small_reduce, dim3(b.count), dim3(64), 0, s, g, b, work, nwg);
+    ACETN_HIP_CHECK(hipGetLastError());
+    return;
   }
   int ks = choose_ksplit(g, b.count);
   if (ks > 1 && (work == nullptr || work_elems < (size_t)ks * b.count * g.M * g.N)) ks = 1;

@@ -752,11 +752,13 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
   __shared__ double Ws[kBjMaxPairs][16 * BJ_LD];
   __shared__ double csw[kBjMaxPairs][8][2];
   __shared__ int pqw[kBjMaxPairs][8];
+  __shared__ int rotw[kBjMaxPairs];  // block pair rotated in this block round
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nb = (q + 7) / 8;
   const int N = nb + (nb & 1);
   const int NP = N / 2;
   const int nstore = N * (N + 1) / 2 * 64;
+  double* vflag = vrec + (int64_t)kBjMaxSweeps * (N - 1) * NP * 256;
   for (int e = tid; e < nstore; e += BJ_NT) S[e] = 0.0;
   __syncthreads();
   for (int e = tid; e < q * q; e += BJ_NT) {
@@ -802,7 +804,21 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
         // local indices >= nI (block I) or >= 8 + nJ (block J) are padding
         const int nI = min(8, max(0, q - 8 * I)), nJ = min(8, max(0, q - 8 * J));
         rep_wave_sync();
-        const int inner = (r == 0) ? 15 : 8;
+        // Does any pair the inner rounds visit (all pairs in the first block
+        // round of a sweep, the cross pairs otherwise) pass the threshold?  If
+        // none does, no rotation happens (W only changes by rotations), so
+        // the rounds are skipped and V_a stays the identity.
+        bool want = false;
+        for (int e = lane; e < 256; e += 64) {
+          const int x = e >> 4, y = e & 15;
+          if (x < y && (r == 0 || (x < 8 && y >= 8))) {
+            const double apq = W[x * BJ_LD + y];
+            want |= apq != 0.0 &&
+                    apq * apq > tol * tol * fabs(W[x * (BJ_LD + 1)] * W[y * (BJ_LD + 1)]);
+          }
+        }
+        const bool rotate = __any(want);
+        const int inner = !rotate ? 0 : (r == 0) ? 15 : 8;
         for (int rr = 0; rr < inner; ++rr) {
           if (lane < 8) {
             int p, qq;
@@ -858,12 +874,18 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
           }
           rep_wave_sync();
         }
-        for (int e = lane; e < 256; e += 64) {
-          const int x = e >> 4, y = e & 15;
-          if (x <= y) S[bj_addr(bj_global(x, I, J), bj_global(y, I, J))] = W[x * BJ_LD + y];
+        if (rotate) {
+          for (int e = lane; e < 256; e += 64) {
+            const int x = e >> 4, y = e & 15;
+            if (x <= y) S[bj_addr(bj_global(x, I, J), bj_global(y, I, J))] = W[x * BJ_LD + y];
+          }
+          double* out = vrec + ((int64_t)(sweep * (N - 1) + r) * NP + wave) * 256;
+          for (int e = lane; e < 256; e += 64) out[e] = V[(e >> 4) * BJ_LD + (e & 15)];
         }
-        double* out = vrec + ((int64_t)(sweep * (N - 1) + r) * NP + wave) * 256;
-        for (int e = lane; e < 256; e += 64) out[e] = V[(e >> 4) * BJ_LD + (e & 15)];
+        if (lane == 0) {
+          rotw[wave] = rotate ? 1 : 0;
+          vflag[(int64_t)(sweep * (N - 1) + r) * NP + wave] = rotate ? 1.0 : 0.0;
+        }
       }
       __syncthreads();
       // ---- phase 2: S_ac <- V_a^T S_ac V_c for every pair of block pairs a < c ----
@@ -871,6 +893,8 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
         int c = 1;
         while (c * (c + 1) / 2 <= u) ++c;
         const int a = u - c * (c - 1) / 2;
+        // V_a^T S_ac V_c = S_ac when neither block pair rotated
+        if (!rotw[a] && !rotw[c]) continue;
         int Ia, Ja, Ic, Jc;
         round_pair(N, r, a, Ia, Ja);
         round_pair(N, r, c, Ic, Jc);
@@ -938,6 +962,7 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_vectors_kernel(
     Vt[i][j] = (row0 + i == j) ? 1.0 : 0.0;
   }
   const int nrounds = info[0] * (N - 1);
+  const double* vflag = vrec + (int64_t)kBjMaxSweeps * (N - 1) * NP * 256;
   const int m = lane & 15, kq = lane >> 4;
   double bnext[4] = {0.0, 0.0, 0.0, 0.0};
   if (wave < NP && nrounds > 0) {
@@ -955,6 +980,8 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_vectors_kernel(
         for (int ks = 0; ks < 4; ++ks)
           bnext[ks] = vrec[((int64_t)(t + 1) * NP + wave) * 256 + (4 * ks + kq) * 16 + m];
       }
+      // block pairs that did not rotate recorded no V_a (identity)
+      if (vflag[(int64_t)t * NP + wave] == 0.0) continue;
       int I, J;
       round_pair(N, t % (N - 1), wave, I, J);
       cb_d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -1126,8 +1153,9 @@ size_t jacobi_record_elems(int q) {
   const int n = q + (q & 1);
   size_t e = (size_t)kMaxSweeps * (n - 1) * (n / 2) * 2;
   if (q <= kBjMaxQ) {
+    // the 16 x 16 V_a of every (sweep, block round, pair), then one flag each
     const int nb = (q + 7) / 8, N = nb + (nb & 1);
-    e = std::max(e, (size_t)kBjMaxSweeps * (N - 1) * (N / 2) * 256);
+    e = std::max(e, (size_t)kBjMaxSweeps * (N - 1) * (N / 2) * 257);
   }
   return e;
 }

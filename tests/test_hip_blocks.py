@@ -32,7 +32,10 @@ def test_sketch_matches_oracle_generator(hip):
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(1, 1, 1, False, False), (37, 53, 29, False, False),
                                          (64, 64, 64, True, False), (130, 70, 300, False, True),
-                                         (100, 146, 5000, True, False), (512, 66, 1024, False, False)])
+                                         (100, 146, 5000, True, False), (512, 66, 1024, False, False),
+                                         # small-output long-K products (per-lane dot kernel)
+                                         (4, 4, 100000, False, False), (3, 7, 5000, True, True),
+                                         (8, 8, 4096, False, True), (1, 5, 70000, True, False)])
 def test_gemm_matches_numpy(hip, M, N, K, ta, tb):
     rng = np.random.default_rng(M * 1000 + N)
     A = rng.standard_normal((K, M) if ta else (M, K))

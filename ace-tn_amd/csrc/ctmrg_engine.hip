@@ -148,12 +148,22 @@ struct HalfOut {
 };
 
 // A product with an enlarged corner as its A operand: from the corner's bf16
-// planes in mixed precision, from the FP64 matrix otherwise.
+// planes in mixed precision, from the FP64 matrix otherwise.  In mixed
+// precision a product whose result only feeds the next corner product as its
+// B operand hands it over as planes (Cout, then that product's Bin): the
+// split-K reduction writes the pieces directly and C is not stored.
 void mm_corner(Ctx& c, const SplitPlanes& P, int M, int N, int K, bool ta, const double* A,
-               int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc) {
+               int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+               const SplitPlanes* Bin = nullptr, SplitPlanes* Cout = nullptr) {
   const GemmDesc g = dense_desc(M, N, K, ta, lda, false, ldb, ldc);
   if (c.run && c.launch && c.split > 0 && P.p != nullptr) {
-    gemm_split_planar(g, P, B, C, c.split, c.gwork, c.gwork_elems, c.st);
+    if (Cout) {
+      Cout->rows = N;
+      Cout->kp = split_planes_kp(M);
+      Cout->pieces = c.split;
+    }
+    gemm_split_planar(g, P, B, C, c.split, c.gwork, c.gwork_elems, c.st,
+                      Bin && Bin->p ? Bin : nullptr, Cout && Cout->p ? Cout : nullptr);
     return;
   }
   gemm(c, g, A, B, C);
@@ -192,6 +202,10 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   q1t.p = c.take(split_planes_elems((int)kk, (int)m));
   q4n.p = c.take(split_planes_elems((int)kk, (int)n));
   q4t.p = c.take(split_planes_elems((int)n, (int)kk));
+  // the range finder's intermediates W (kk x q) and Z (n x q) as B planes
+  SplitPlanes wp, zp;
+  wp.p = c.take(split_planes_elems((int)q, (int)kk));
+  zp.p = c.take(split_planes_elems((int)q, (int)n));
 
   const bool replay = c.run && phase == 2;
   if (replay) c.launch = false;
@@ -215,15 +229,15 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   // range finder (bf16x3); later products carry the basis itself (bf16x6)
   SplitScope sc(c, 2);
   // Y = Q1 (Q4 Omega)
-  mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Om, q, W, q);
-  mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q);
+  mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Om, q, W, q, nullptr, &wp);
+  mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q, &wp);
   for (int it = 0; it < niter; ++it) {
     if (it >= 1) c.split = c.mixed ? 3 : 0;
     orthonormalize(c, Y, m, q);
-    mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Y, q, W, q);   // Q1^H Y
-    mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, W, q, Z, q);    // Q4^H (.)
-    mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Z, q, W, q);   // Q4 (.)
-    mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q);  // Q1 (.)
+    mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Y, q, W, q, nullptr, &wp);  // Q1^H Y
+    mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, W, q, Z, q, &wp, &zp);       // Q4^H (.)
+    mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Z, q, W, q, &zp, &wp);      // Q4 (.)
+    mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q, &wp);          // Q1 (.)
   }
   c.split = c.mixed ? 3 : 0;
   orthonormalize(c, Y, m, q);

@@ -36,6 +36,7 @@ struct DLShape {
   static constexpr int ROWS = NB * D2;            // stacked rows
   static constexpr int MP = (ROWS + 15) / 16 * 16; // padded rows (step c M, step d N)
   static constexpr int KP = (D2 + 3) / 4 * 4;      // padded contraction length
+  static constexpr int XLD = KP + 1;               // odd LDS row stride of the X tile (no bank conflicts)
   static constexpr int NP = (D2 + 15) / 16 * 16;   // padded (b3,b4) columns / step d rows
   static constexpr int TC = (MP / 16) * (NP / 16); // tiles per product
   static constexpr int TPW = (TC + NW - 1) / NW;   // tiles per wave
@@ -50,8 +51,8 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
   extern __shared__ double sm[];
   const int d = g.d;
   double* Ac = sm;                          // [p][KP][NP]: A[(l1,l2),(l3,l4),p], zero padded
-  double* Xs = Ac + (size_t)d * S::KP * S::NP;  // [MP][KP]: stacked blocks, rows (blk,k1,k2)
-  double* Ts = Xs + S::MP * S::KP;              // [MP][NP]: rows (blk,k1,k2), cols (b3,b4)
+  double* Xs = Ac + (size_t)d * S::KP * S::NP;  // [MP][XLD]: stacked blocks, rows (blk,k1,k2)
+  double* Ts = Xs + S::MP * S::XLD;             // [MP][NP]: rows (blk,k1,k2), cols (b3,b4)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   for (int e = tid; e < d * S::KP * S::NP; e += NT) {
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
 #pragma unroll
     for (int t = 0; t < S::XPT; ++t) {
       const int e = tid + t * NT;
-      if (e < S::MP * S::KP) Xs[e] = xr[t];
+      if (e < S::MP * S::KP) Xs[(e / S::KP) * S::XLD + e % S::KP] = xr[t];
     }
     if (grp + (int)gridDim.x < ngroups) load_x(grp + gridDim.x);
     d4 yacc[S::TPW];
@@ -122,13 +123,13 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
       // ---- step c: Ts[MP x NP] = Xs[MP x KP] . Abar_pb[KP x NP]
 #pragma unroll
       for (int t = 0; t < S::TPW; ++t) {
-        const int tile = wave * S::TPW + t;
+        const int tile = t * NW + wave;  // round-robin over the waves (even SIMD load)
         if (tile < S::TC) {
           const int tm = tile / (S::NP / 16), tn = tile % (S::NP / 16);
           d4 acc = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 3
           for (int ks = 0; ks < S::KP; ks += 4) {
-            const double a = Xs[(tm * 16 + (lane & 15)) * S::KP + ks + (lane >> 4)];
+            const double a = Xs[(tm * 16 + (lane & 15)) * S::XLD + ks + (lane >> 4)];
             const double b = Apb[(ks + (lane >> 4)) * S::NP + tn * 16 + (lane & 15)];
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
           }
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
         // ---- step d: Y[(k3,k4), (blk,b3,b4)] += A_p^T[(k3,k4),(k1,k2)] . T[(blk,k1,k2),(b3,b4)]
 #pragma unroll
         for (int t = 0; t < S::TPW; ++t) {
-          const int tile = wave * S::TPW + t;
+          const int tile = t * NW + wave;  // round-robin over the waves (even SIMD load)
           if (tile < S::TC) {
             const int tm = tile / (S::MP / 16), tn = tile % (S::MP / 16);
             const int nn = tn * 16 + (lane & 15);
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
           // ---- write Y (after the last p when closed, per (p, pb) when open)
 #pragma unroll
           for (int t = 0; t < S::TPW; ++t) {
-            const int tile = wave * S::TPW + t;
+            const int tile = t * NW + wave;  // round-robin over the waves (even SIMD load)
             if (tile < S::TC) {
               const int tm = tile / (S::MP / 16), tn = tile % (S::MP / 16);
               const int nn = tn * 16 + (lane & 15);
@@ -196,7 +197,7 @@ template <int D>
 void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hipStream_t s) {
   using S = DLShape<D>;
   static_assert(S::TC == (S::NP / 16) * (S::MP / 16), "tile counts of the two products differ");
-  const size_t lds = sizeof(double) * ((size_t)g.d * S::KP * S::NP + (size_t)S::MP * S::KP +
+  const size_t lds = sizeof(double) * ((size_t)g.d * S::KP * S::NP + (size_t)S::MP * S::XLD +
                                        (size_t)S::MP * S::NP);
   ACETN_REQUIRE(lds <= 160 * 1024, "double_layer: LDS budget exceeded");
   ACETN_REQUIRE((int64_t)(D - 1) * (g.x_k1 + g.x_b1 + g.x_k2 + g.x_b2) < (int64_t(1) << 31),

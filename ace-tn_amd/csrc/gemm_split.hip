@@ -471,6 +471,60 @@ __global__ void splitk_reduce_f32(GemmDesc g, double* __restrict__ C, int ksplit
   }
 }
 
+// The FP32 split-K partials of C (M x N, row-major) reduced in the order of
+// splitk_reduce_f32 and written as the bf16 planes [piece][n][kp] of a
+// following product's B operand (B(k, n) = alpha C(k, n), zero for
+// M <= k < kp): the same pieces the FP64 C would be split into, without
+// storing it.  64 x 64 tiles transposed through LDS (partials read along n,
+// planes written along k).
+template <int SPLIT>
+__global__ __launch_bounds__(256) void splitk_reduce_planes_kernel(int M, int N, int ksplit,
+                                                                   double alpha,
+                                                                   const float* __restrict__ work,
+                                                                   int Kp, __bf16* __restrict__ planes) {
+  constexpr int EPT = 64 * 64 / 256;  // elements per thread
+  __shared__ float tile[64][65];
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int64_t mn = (int64_t)M * N;
+  // all EPT elements' loads of one part in flight together; each element
+  // still sums its parts in ascending order
+  double acc[EPT];
+  int64_t off[EPT];
+  bool ok[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int e = threadIdx.x + j * 256;
+    const int m = m0 + e / 64, n = n0 + e % 64;
+    ok[j] = m < M && n < N;
+    off[j] = ok[j] ? (int64_t)m * N + n : 0;
+    acc[j] = 0.0;
+  }
+  for (int p = 0; p < ksplit; ++p) {
+    float x[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) x[j] = work[(int64_t)p * mn + off[j]];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) acc[j] += (double)x[j];
+  }
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int e = threadIdx.x + j * 256;
+    tile[e / 64][e % 64] = ok[j] ? (float)(alpha * acc[j]) : 0.f;
+  }
+  __syncthreads();
+  const int64_t plane = (int64_t)N * Kp;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int ni = e / 64, mi = e % 64;
+    const int m = m0 + mi, n = n0 + ni;
+    if (n < N && m < Kp) {
+      __bf16 pc[SPLIT];
+      split<SPLIT>(tile[mi][ni], pc);
+#pragma unroll
+      for (int s2 = 0; s2 < SPLIT; ++s2) planes[s2 * plane + (int64_t)n * Kp + m] = pc[s2];
+    }
+  }
+}
+
 // general-path tiles: 128 x 160 for large products, 64 x 64 otherwise
 struct GTile {
   int wm, wn;
@@ -602,7 +656,7 @@ size_t plane_elems(const GemmDesc& g, int split) {
 
 template <int WM, int WN, int SPLIT>
 void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* Bp, int Kp,
-            double* C, int ks, double* work, hipStream_t s) {
+            double* C, int ks, double* work, hipStream_t s, SplitPlanes* Cp = nullptr) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
   const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
   planar_wgs_per_cu<WM, WN, SPLIT>();  // sets the dynamic-LDS attribute once
@@ -614,7 +668,17 @@ void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* B
                      Kp, C, ks, wf);
   ACETN_HIP_CHECK(hipGetLastError());
   kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
-  if (ks > 1) {
+  if (ks > 1 && Cp != nullptr) {
+    const dim3 grid((Cp->kp + 63) / 64, (g.N + 63) / 64);
+    __bf16* out = reinterpret_cast<__bf16*>(Cp->p);
+    if (Cp->pieces == 2)
+      hipLaunchKernelGGL(splitk_reduce_planes_kernel<2>, grid, dim3(256), 0, s, g.M, g.N, ks,
+                         g.alpha, wf, Cp->kp, out);
+    else
+      hipLaunchKernelGGL(splitk_reduce_planes_kernel<3>, grid, dim3(256), 0, s, g.M, g.N, ks,
+                         g.alpha, wf, Cp->kp, out);
+    ACETN_HIP_CHECK(hipGetLastError());
+  } else if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     const int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
     hipLaunchKernelGGL(splitk_reduce_f32, dim3(blocks), dim3(256), 0, s, g, C, ks, wf);
@@ -623,21 +687,28 @@ void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* B
 }
 
 // C = A B with A given as planes; B (FP64 through its index maps) is split
-// into planes after the partials in the scratch.
+// into planes after the partials in the scratch, unless Bin already holds
+// them.  With Cp the result goes to planes (see gemm_split_planar).
 template <int SPLIT>
 void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, double* C, int ks,
-         double* work, hipStream_t s) {
+         double* work, hipStream_t s, const SplitPlanes* Bin = nullptr, SplitPlanes* Cp = nullptr) {
   const int Kp = padded_k(g.K);
   ACETN_REQUIRE((int64_t)SPLIT * aplane < (int64_t(1) << 31) &&
                     (int64_t)SPLIT * g.N * Kp < (int64_t(1) << 31),
                 "gemm_split: operand planes exceed 2^31 elements");
   __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, ks));
-  split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
+  if (Bin != nullptr) {
+    ACETN_REQUIRE(Bin->p && Bin->rows == g.N && Bin->kp == Kp && Bin->pieces >= SPLIT,
+                  "gemm_split: B planes do not match the product");
+    planes = reinterpret_cast<__bf16*>(Bin->p);
+  } else {
+    split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
+  }
   const Tile t = choose_tile(g);
-  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
-  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
-  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
-  else launch<2, 2, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s);
+  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
+  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
+  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
+  else launch<2, 2, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
 }
 }  // namespace
 
@@ -683,7 +754,8 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
 }
 
 void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B, double* C,
-                       int split, double* work, size_t work_elems, hipStream_t s) {
+                       int split, double* work, size_t work_elems, hipStream_t s,
+                       const SplitPlanes* Bp, SplitPlanes* Cp) {
   ACETN_REQUIRE(split == 2 || split == 3, "gemm_split: split must be 2 or 3");
   ACETN_REQUIRE(A.p != nullptr && A.rows == g.M && A.kp == padded_k(g.K) && A.pieces >= split,
                 "gemm_split: A planes do not match the product");
@@ -692,11 +764,30 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
                 "gemm_split: scratch too small");
   int ks = choose_ksplit_run(g, split);
   if (work_elems < partial_elems(g, ks) + plane_elems(g, split)) ks = 1;
+  if (Cp != nullptr) {
+    ACETN_REQUIRE(g.beta == 0.0 && Cp->p && Cp->rows == g.N && Cp->kp == padded_k(g.M) &&
+                      (Cp->pieces == 2 || Cp->pieces == 3),
+                  "gemm_split: C planes do not match the product");
+    // without split-K partials the product writes C: split it afterwards
+    if (ks == 1) {
+      const int64_t aplane = (int64_t)A.rows * A.kp;
+      const __bf16* Ap = reinterpret_cast<const __bf16*>(A.p);
+      if (split == 2) run<2>(g, Ap, aplane, B, C, ks, work, s, Bp);
+      else run<3>(g, Ap, aplane, B, C, ks, work, s, Bp);
+      // planes of C(k = m, n) = C[cm(m) + cn(n)]
+      __bf16* out = reinterpret_cast<__bf16*>(Cp->p);
+      if (Cp->pieces == 2) split_into_planes<2>(C, g.cm, g.cn, g.M, g.N, Cp->kp, out, s);
+      else split_into_planes<3>(C, g.cm, g.cn, g.M, g.N, Cp->kp, out, s);
+      return;
+    }
+  }
   const int64_t aplane = (int64_t)A.rows * A.kp;
   const __bf16* Ap = reinterpret_cast<const __bf16*>(A.p);
-  if (split == 2) run<2>(g, Ap, aplane, B, C, ks, work, s);
-  else run<3>(g, Ap, aplane, B, C, ks, work, s);
+  if (split == 2) run<2>(g, Ap, aplane, B, C, ks, work, s, Bp, Cp);
+  else run<3>(g, Ap, aplane, B, C, ks, work, s, Bp, Cp);
 }
+
+int split_planes_kp(int K) { return padded_k(K); }
 
 size_t split_planes_elems(int rows, int K) {
   const size_t bytes = (size_t)3 * rows * padded_k(K) * sizeof(__bf16);

@@ -113,14 +113,20 @@ struct SplitPlanes {
   int rows = 0, kp = 0, pieces = 0;
 };
 size_t split_planes_elems(int rows, int K);
+int split_planes_kp(int K);  // kp of K
 // Both orientations of X (R x Cc, row-major, leading dimension ld): N holds X
 // itself (rows R, k = columns), T holds X^T (rows Cc, k = rows of X).
 void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N, SplitPlanes& T,
                      hipStream_t s);
 // C = alpha A B + beta C with A given as planes (g.am/g.ak unused; A(m, k) =
-// the planes' row m, column k).
+// the planes' row m, column k).  Bp, when given, holds B already split into
+// planes of the next product's layout (rows N, k = K; g.bk/g.bn unused).
+// Cp, when given (beta = 0), receives C as such planes (rows N, k = M, for a
+// following product that takes C as its B operand, at that product's split
+// level Cp->pieces) in place of the FP64 C.
 void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B, double* C,
-                       int split, double* work, size_t work_elems, hipStream_t s);
+                       int split, double* work, size_t work_elems, hipStream_t s,
+                       const SplitPlanes* Bp = nullptr, SplitPlanes* Cp = nullptr);
 
 // Live timing of the matrix-core kernels (HIP events on the launch stream),
 // per kernel class; the benchmark's roofline figures come from here.

@@ -37,6 +37,7 @@ def _declare(lib):
         "acetn_hip_last_error": (ctypes.c_char_p, []),
         "acetn_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
         "acetn_hip_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
+        "acetn_hip_kernel_timing_bytes": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_dbl)]),
         "acetn_hip_kernel_timing_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_dbl),
                                                         ctypes.POINTER(c_dbl),
                                                         ctypes.POINTER(c_i64)]),
@@ -295,6 +296,13 @@ def kernel_timing_read(kernel=K_GEMM_F64):
     check(lib().acetn_hip_kernel_timing_read(kernel, ctypes.byref(ms), ctypes.byref(fl),
                                              ctypes.byref(n)))
     return ms.value, fl.value, n.value
+
+
+def kernel_timing_bytes(kernel=K_GEMM_F64):
+    """Summed algorithmic HBM bytes of one kernel class."""
+    b = c_dbl(0)
+    check(lib().acetn_hip_kernel_timing_bytes(kernel, ctypes.byref(b)))
+    return b.value
 
 
 # ---------------------------------------------------------------- wrappers

@@ -766,6 +766,10 @@ int acetn_hip_kernel_timing(int enable) {
   return guarded([&] { kernel_timing_enable(enable != 0); });
 }
 
+int acetn_hip_kernel_timing_bytes(int kernel, double* host_bytes) {
+  return guarded([&] { *host_bytes = kernel_timing_bytes(kernel); });
+}
+
 int acetn_hip_kernel_timing_read(int kernel, double* host_ms, double* host_flops,
                                  int64_t* host_launches) {
   return guarded([&] { kernel_timing_read(kernel, host_ms, host_flops, host_launches); });

@@ -220,7 +220,9 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
   hipEvent_t e0 = kernel_timing_begin(s);
   hipLaunchKernelGGL(dl_mfma_kernel<D>, dim3(grid), dim3(NT), lds, s, g, X, A, Y);
   ACETN_HIP_CHECK(hipGetLastError());
-  kernel_timing_end(e0, s, kKDoubleLayer, flops);
+  // algorithmic bytes: the X blocks read once, the Y blocks written once
+  const double yblk = g.open_p ? (double)g.d * g.d : 1.0;
+  kernel_timing_end(e0, s, kKDoubleLayer, flops, 8.0 * (double)nblk * S::D2 * S::D2 * (1.0 + yblk));
 }
 }  // namespace
 

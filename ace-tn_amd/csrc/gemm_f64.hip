@@ -31,6 +31,7 @@ struct KTimer {
   struct Rec { hipEvent_t e0, e1; int cls; };
   std::vector<Rec> ev;
   double flops[kKClasses] = {};
+  double bytes[kKClasses] = {};
   int64_t launches[kKClasses] = {};
 };
 KTimer& ktimer() {
@@ -44,7 +45,7 @@ void kernel_timing_enable(bool on) {
   std::lock_guard<std::mutex> g(t.mu);
   for (auto& e : t.ev) { (void)hipEventDestroy(e.e0); (void)hipEventDestroy(e.e1); }
   t.ev.clear();
-  for (int c = 0; c < kKClasses; ++c) { t.flops[c] = 0.0; t.launches[c] = 0; }
+  for (int c = 0; c < kKClasses; ++c) { t.flops[c] = 0.0; t.bytes[c] = 0.0; t.launches[c] = 0; }
   t.on = on;
 }
 
@@ -74,7 +75,14 @@ hipEvent_t kernel_timing_begin(hipStream_t s) {
   return e0;
 }
 
-void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops) {
+double kernel_timing_bytes(int cls) {
+  ACETN_REQUIRE(cls >= 0 && cls < kKClasses, "kernel_timing_bytes: unknown kernel class");
+  KTimer& t = ktimer();
+  std::lock_guard<std::mutex> g(t.mu);
+  return t.bytes[cls];
+}
+
+void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops, double bytes) {
   if (e0 == nullptr) return;
   KTimer& t = ktimer();
   hipEvent_t e1 = nullptr;
@@ -83,6 +91,7 @@ void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops) {
   std::lock_guard<std::mutex> lg(t.mu);
   t.ev.push_back({e0, e1, cls});
   t.flops[cls] += flops;
+  t.bytes[cls] += bytes;
   t.launches[cls] += 1;
 }
 
@@ -351,16 +360,27 @@ __global__ __launch_bounds__(kDotNT) void dot_small_kernel(GemmDesc g, GemmBatch
   }
 }
 
-__global__ void dot_small_reduce(GemmDesc g, GemmBatch bt, const double* __restrict__ part,
-                                 int nwg) {
+// one workgroup per output element: strided partial sums, then a fixed
+// binary tree in LDS (deterministic)
+__global__ __launch_bounds__(kDotNT) void dot_small_reduce(GemmDesc g, GemmBatch bt,
+                                                           const double* __restrict__ part,
+                                                           int nwg) {
   constexpr int MN = kDotMax * kDotMax;
+  __shared__ double red[kDotNT];
   if (g.run_if && *g.run_if == 0) return;
-  const int prob = blockIdx.x, t = threadIdx.x;
-  if (t >= g.M * g.N) return;
-  const int m = t / g.N, n = t % g.N;
+  const int prob = blockIdx.y, t = threadIdx.x;
+  const int m = blockIdx.x / g.N, n = blockIdx.x % g.N;
   const double* p = part + (int64_t)prob * nwg * MN + m * kDotMax + n;
   double s = 0.0;
-  for (int w = 0; w < nwg; ++w) s += p[(int64_t)w * MN];
+  for (int w = t; w < nwg; w += kDotNT) s += p[(int64_t)w * MN];
+  red[t] = s;
+  __syncthreads();
+  for (int o = kDotNT / 2; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  s = red[0];
   double* C = bt.C[prob];
   const int64_t off = imap(g.cm, m) + imap(g.cn, n);
   C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
@@ -421,8 +441,10 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
     hipEvent_t e0 = kernel_timing_begin(s);
     hipLaunchKernelGGL(dot_small_kernel, dim3(nwg, b.count), dim3(kDotNT), 0, s, g, b, work);
     ACETN_HIP_CHECK(hipGetLastError());
-    kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count);
-    hipLaunchKernelGGL(dot_small_reduce, dim3(b.count), dim3(64), 0, s, g, b, work, nwg);
+    kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count,
+                      8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N) * b.count);
+    hipLaunchKernelGGL(dot_small_reduce, dim3(g.M * g.N, b.count), dim3(kDotNT), 0, s, g, b, work,
+                       nwg);
     ACETN_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -435,7 +457,8 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
   else if (t.wm == 4 && t.wn == 4) launch_gemm<4, 4>(g, b, ks, work, s);
   else launch_gemm<2, 2>(g, b, ks, work, s);
   ACETN_HIP_CHECK(hipGetLastError());
-  kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count);
+  kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count,
+                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N) * b.count);
   if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);

@@ -565,7 +565,9 @@ void launch_general(const GemmDesc& g, const double* A, const double* B, double*
   hipLaunchKernelGGL((gemm_general_kernel<SPLIT, WM, WN>), grid, dim3(NT8), lds, s, g, A, B, C, ks,
                      wf);
   ACETN_HIP_CHECK(hipGetLastError());
-  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
+  // algorithmic bytes: FP64 operands read once, C written once
+  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K,
+                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
   if (ks > 1) {
     const int64_t mn = (int64_t)g.M * g.N;
     const int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
@@ -667,7 +669,9 @@ void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* B
   hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT>), grid, dim3(NT), lds, s, g, Ap, aplane, Bp,
                      Kp, C, ks, wf);
   ACETN_HIP_CHECK(hipGetLastError());
-  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K);
+  // algorithmic bytes: both operands as SPLIT bf16 planes, C once in FP64
+  kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K,
+                    2.0 * SPLIT * ((double)g.M + g.N) * Kp + 8.0 * g.M * g.N);
   if (ks > 1 && Cp != nullptr) {
     const dim3 grid((Cp->kp + 63) / 64, (g.N + 63) / 64);
     __bf16* out = reinterpret_cast<__bf16*>(Cp->p);

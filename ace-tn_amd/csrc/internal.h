@@ -136,7 +136,9 @@ void kernel_timing_read(int cls, double* ms, double* flops, int64_t* launches);
 // Returns an event recorded on ``s`` (nullptr when timing is off); pass it to
 // kernel_timing_end after the timed launch.
 hipEvent_t kernel_timing_begin(hipStream_t s);
-void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops);
+void kernel_timing_end(hipEvent_t e0, hipStream_t s, int cls, double flops, double bytes = 0.0);
+// summed algorithmic HBM bytes (operands read once, result written once) of a class
+double kernel_timing_bytes(int cls);
 
 // Convenience: dense row-major C = alpha*op(A) op(B) + beta*C.
 // A is (M x K) if !ta else stored (K x M); lda/ldb/ldc are row strides.

@@ -73,6 +73,11 @@ enum { ACETN_HIP_K_GEMM_F64 = 0, ACETN_HIP_K_GEMM_SPLIT = 1, ACETN_HIP_K_DOUBLE_
 int acetn_hip_kernel_timing(int enable);
 int acetn_hip_kernel_timing_read(int kernel, double* host_ms, double* host_flops,
                                  int64_t* host_launches);
+/* Summed algorithmic HBM bytes of one kernel class since the last reset:
+ * operands read once and the result written once, in the formats the class
+ * moves them in (FP64, or the bf16 planes of ACETN_HIP_K_GEMM_SPLIT's
+ * products from planes).  No counterpart in the reference (measurement). */
+int acetn_hip_kernel_timing_bytes(int kernel, double* host_bytes);
 
 /* Enlarged corner ("quarter tensor") Q_k of one site, as an
  * (C_dim[k][1]*D^2) x (E_dim[(3+k)%4][0]*D^2) matrix.

@@ -73,7 +73,9 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
   // Y = Q viewed as [c][r][R][e][d][D]
   dl.y_o1 = D2 * qd.N2; dl.y_k3 = D * qd.N2; dl.y_b3 = qd.N2;
   dl.y_o2 = D2; dl.y_k4 = D; dl.y_b4 = 1;
-  double_layer_f64(dl, T2, s->A, Q, c.st);
+  // mixed precision: the absorption in FP32 on the matrix cores
+  if (c.mixed) double_layer_f32(dl, T2, s->A, Q, c.st);
+  else double_layer_f64(dl, T2, s->A, Q, c.st);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,7 +542,8 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
       const int64_t rowz = xa * D2;
       dl.y_o2 = D2 * rowz; dl.y_k3 = D * rowz; dl.y_b3 = rowz;
       dl.y_o1 = D2; dl.y_k4 = D; dl.y_b4 = 1;
-      double_layer_f64(dl, T, s1->A, Zt, c.st);
+      if (c.mixed) double_layer_f32(dl, T, s1->A, Zt, c.st);
+      else double_layer_f64(dl, T, s1->A, Zt, c.st);
     }
     GemmDesc h{};
     h.M = (int)(cDj * D2); h.N = (int)cDi; h.K = (int)(xa * D2);

@@ -21,6 +21,28 @@
 namespace acetn {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Compute type of the double-layer products: FP64 (v_mfma_f64_16x16x4f64)
+// or FP32 (v_mfma_f32_16x16x4f32, the mixed-precision path).  Operands of
+// both: A(16x4) lane l holds A[l%16][l/16], B(4x16) lane l holds
+// B[l/16][l%16].  Results: the FP64 16x16 tile holds D[l/16 + 4r][l%16] in
+// element r of lane l, the FP32 one D[4(l/16) + r][l%16].
+template <typename T> struct DLMath;
+template <> struct DLMath<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct DLMath<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
 
 namespace {
 // 8 waves per workgroup: the LDS footprint (A, the stacked X tile and the
@@ -43,16 +65,19 @@ struct DLShape {
   static constexpr int XPT = (MP * KP + NT - 1) / NT; // X elements per thread
 };
 
-template <int D>
+template <int D, typename T>
 __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __restrict__ X,
                                                      const double* __restrict__ A,
                                                      double* __restrict__ Y) {
   using S = DLShape<D>;
-  extern __shared__ double sm[];
+  using M = DLMath<T>;
+  typedef typename M::acc_t acc_t;
+  extern __shared__ __align__(16) char smem[];
+  T* sm = reinterpret_cast<T*>(smem);
   const int d = g.d;
-  double* Ac = sm;                          // [p][KP][NP]: A[(l1,l2),(l3,l4),p], zero padded
-  double* Xs = Ac + (size_t)d * S::KP * S::NP;  // [MP][XLD]: stacked blocks, rows (blk,k1,k2)
-  double* Ts = Xs + S::MP * S::XLD;             // [MP][NP]: rows (blk,k1,k2), cols (b3,b4)
+  T* Ac = sm;                               // [p][KP][NP]: A[(l1,l2),(l3,l4),p], zero padded
+  T* Xs = Ac + (size_t)d * S::KP * S::NP;   // [MP][XLD]: stacked blocks, rows (blk,k1,k2)
+  T* Ts = Xs + S::MP * S::XLD;              // [MP][NP]: rows (blk,k1,k2), cols (b3,b4)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   for (int e = tid; e < d * S::KP * S::NP; e += NT) {
@@ -63,7 +88,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
       const int l1 = kk / D, l2 = kk % D, l3 = nn / D, l4 = nn % D;
       v = A[l1 * g.a_leg[0] + l2 * g.a_leg[1] + l3 * g.a_leg[2] + l4 * g.a_leg[3] + p * g.a_p];
     }
-    Ac[e] = v;
+    Ac[e] = (T)v;
   }
 
   const int nblk = g.n1 * g.n2;
@@ -109,16 +134,16 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
 #pragma unroll
     for (int t = 0; t < S::XPT; ++t) {
       const int e = tid + t * NT;
-      if (e < S::MP * S::KP) Xs[(e / S::KP) * S::XLD + e % S::KP] = xr[t];
+      if (e < S::MP * S::KP) Xs[(e / S::KP) * S::XLD + e % S::KP] = (T)xr[t];
     }
     if (grp + (int)gridDim.x < ngroups) load_x(grp + gridDim.x);
-    d4 yacc[S::TPW];
+    acc_t yacc[S::TPW];
 #pragma unroll
-    for (int t = 0; t < S::TPW; ++t) yacc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < S::TPW; ++t) yacc[t] = acc_t{0, 0, 0, 0};
 
     // closed: sum_p A_p^T (X Abar_p);  open: every (p, pb) pair written out
     for (int pb = 0; pb < d; ++pb) {
-      const double* Apb = Ac + (size_t)pb * S::KP * S::NP;
+      const T* Apb = Ac + (size_t)pb * S::KP * S::NP;
       __syncthreads();
       // ---- step c: Ts[MP x NP] = Xs[MP x KP] . Abar_pb[KP x NP]
 #pragma unroll
@@ -126,24 +151,24 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
         const int tile = t * NW + wave;  // round-robin over the waves (even SIMD load)
         if (tile < S::TC) {
           const int tm = tile / (S::NP / 16), tn = tile % (S::NP / 16);
-          d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+          acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll 3
           for (int ks = 0; ks < S::KP; ks += 4) {
-            const double a = Xs[(tm * 16 + (lane & 15)) * S::XLD + ks + (lane >> 4)];
-            const double b = Apb[(ks + (lane >> 4)) * S::NP + tn * 16 + (lane & 15)];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            const T a = Xs[(tm * 16 + (lane & 15)) * S::XLD + ks + (lane >> 4)];
+            const T b = Apb[(ks + (lane >> 4)) * S::NP + tn * 16 + (lane & 15)];
+            acc = M::mfma(a, b, acc);
           }
           for (int r = 0; r < 4; ++r)
-            Ts[(tm * 16 + (lane >> 4) + 4 * r) * S::NP + tn * 16 + (lane & 15)] = acc[r];
+            Ts[(tm * 16 + M::row(lane, r)) * S::NP + tn * 16 + (lane & 15)] = acc[r];
         }
       }
       __syncthreads();
       const int p_lo = g.open_p ? 0 : pb, p_hi = g.open_p ? d : pb + 1;
       for (int p = p_lo; p < p_hi; ++p) {
-        const double* Ap = Ac + (size_t)p * S::KP * S::NP;
+        const T* Ap = Ac + (size_t)p * S::KP * S::NP;
         if (g.open_p) {
 #pragma unroll
-          for (int t = 0; t < S::TPW; ++t) yacc[t] = d4{0.0, 0.0, 0.0, 0.0};
+          for (int t = 0; t < S::TPW; ++t) yacc[t] = acc_t{0, 0, 0, 0};
         }
         // ---- step d: Y[(k3,k4), (blk,b3,b4)] += A_p^T[(k3,k4),(k1,k2)] . T[(blk,k1,k2),(b3,b4)]
 #pragma unroll
@@ -156,10 +181,9 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
 #pragma unroll 3
             for (int ks = 0; ks < S::KP; ks += 4) {
               const int kk = ks + (lane >> 4);
-              const double a = Ap[kk * S::NP + tm * 16 + (lane & 15)];
-              const double b =
-                  (kk < S::D2 && nn < S::ROWS) ? Ts[(blk * S::D2 + kk) * S::NP + b34] : 0.0;
-              yacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, yacc[t], 0, 0, 0);
+              const T a = Ap[kk * S::NP + tm * 16 + (lane & 15)];
+              const T b = (kk < S::D2 && nn < S::ROWS) ? Ts[(blk * S::D2 + kk) * S::NP + b34] : T(0);
+              yacc[t] = M::mfma(a, b, yacc[t]);
             }
           }
         }
@@ -178,10 +202,10 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
                 const int64_t yb = (int64_t)o1 * g.y_o1 + (int64_t)o2 * g.y_o2 + poff;
                 const int b3 = b34 / D, b4 = b34 % D;
                 for (int r = 0; r < 4; ++r) {
-                  const int m = tm * 16 + (lane >> 4) + 4 * r;
+                  const int m = tm * 16 + M::row(lane, r);
                   if (m < S::D2) {
                     const int k3 = m / D, k4 = m % D;
-                    Y[yb + k3 * g.y_k3 + b3 * g.y_b3 + k4 * g.y_k4 + b4 * g.y_b4] = yacc[t][r];
+                    Y[yb + k3 * g.y_k3 + b3 * g.y_b3 + k4 * g.y_k4 + b4 * g.y_b4] = (double)yacc[t][r];
                   }
                 }
               }
@@ -193,18 +217,18 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
   }
 }
 
-template <int D>
+template <int D, typename T>
 void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hipStream_t s) {
   using S = DLShape<D>;
   static_assert(S::TC == (S::NP / 16) * (S::MP / 16), "tile counts of the two products differ");
-  const size_t lds = sizeof(double) * ((size_t)g.d * S::KP * S::NP + (size_t)S::MP * S::XLD +
-                                       (size_t)S::MP * S::NP);
+  const size_t lds = sizeof(T) * ((size_t)g.d * S::KP * S::NP + (size_t)S::MP * S::XLD +
+                                  (size_t)S::MP * S::NP);
   ACETN_REQUIRE(lds <= 160 * 1024, "double_layer: LDS budget exceeded");
   ACETN_REQUIRE((int64_t)(D - 1) * (g.x_k1 + g.x_b1 + g.x_k2 + g.x_b2) < (int64_t(1) << 31),
                 "double_layer: block strides exceed 32-bit offsets");
   static bool attr = false;
   if (!attr) {
-    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)dl_mfma_kernel<D>,
+    ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)dl_mfma_kernel<D, T>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
@@ -218,7 +242,7 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
   const double d6 = (double)S::D2 * S::D2 * S::D2;
   const double flops = (double)nblk * (2.0 * g.d * d6 + 2.0 * (g.open_p ? g.d * g.d : g.d) * d6);
   hipEvent_t e0 = kernel_timing_begin(s);
-  hipLaunchKernelGGL(dl_mfma_kernel<D>, dim3(grid), dim3(NT), lds, s, g, X, A, Y);
+  hipLaunchKernelGGL((dl_mfma_kernel<D, T>), dim3(grid), dim3(NT), lds, s, g, X, A, Y);
   ACETN_HIP_CHECK(hipGetLastError());
   // algorithmic bytes: the X blocks read once, the Y blocks written once
   const double yblk = g.open_p ? (double)g.d * g.d : 1.0;
@@ -226,21 +250,33 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
 }
 }  // namespace
 
-void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
-                      hipStream_t s) {
+namespace {
+template <typename T>
+void double_layer_t(const DLDesc& g, const double* X, const double* A, double* Y, hipStream_t s) {
   ACETN_REQUIRE(g.d >= 1 && g.d <= 4, "double_layer: d must be in [1, 4]");
   if ((int64_t)g.n1 * g.n2 <= 0) return;
   switch (g.D) {
-    case 1: launch_dl<1>(g, X, A, Y, s); break;
-    case 2: launch_dl<2>(g, X, A, Y, s); break;
-    case 3: launch_dl<3>(g, X, A, Y, s); break;
-    case 4: launch_dl<4>(g, X, A, Y, s); break;
-    case 5: launch_dl<5>(g, X, A, Y, s); break;
-    case 6: launch_dl<6>(g, X, A, Y, s); break;
-    case 7: launch_dl<7>(g, X, A, Y, s); break;
-    case 8: launch_dl<8>(g, X, A, Y, s); break;
+    case 1: launch_dl<1, T>(g, X, A, Y, s); break;
+    case 2: launch_dl<2, T>(g, X, A, Y, s); break;
+    case 3: launch_dl<3, T>(g, X, A, Y, s); break;
+    case 4: launch_dl<4, T>(g, X, A, Y, s); break;
+    case 5: launch_dl<5, T>(g, X, A, Y, s); break;
+    case 6: launch_dl<6, T>(g, X, A, Y, s); break;
+    case 7: launch_dl<7, T>(g, X, A, Y, s); break;
+    case 8: launch_dl<8, T>(g, X, A, Y, s); break;
     default: ACETN_REQUIRE(false, "double_layer: D must be in [1, 8]");
   }
+}
+}  // namespace
+
+void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
+                      hipStream_t s) {
+  double_layer_t<double>(g, X, A, Y, s);
+}
+
+void double_layer_f32(const DLDesc& g, const double* X, const double* A, double* Y,
+                      hipStream_t s) {
+  double_layer_t<float>(g, X, A, Y, s);
 }
 
 }  // namespace acetn

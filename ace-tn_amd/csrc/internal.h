@@ -165,6 +165,10 @@ struct DLDesc {
 };
 void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
                       hipStream_t s);
+// The same contraction with FP32 operands, products and accumulation on the
+// matrix cores (v_mfma_f32_16x16x4f32), FP64 in and out: the mixed-precision path.
+void double_layer_f32(const DLDesc& g, const double* X, const double* A, double* Y,
+                      hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // small dense linear algebra (one workgroup per problem)

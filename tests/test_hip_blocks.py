@@ -137,3 +137,23 @@ def test_quarter_tensor_matches_oracle(hip, D, chi, k):
     Qh = hip.quarter_tensor((dev(A), [dev(c) for c in C], [dev(e) for e in E]), k).cpu().numpy()
     Qh = Qh / np.abs(Qh).max()
     np.testing.assert_allclose(Qh, Qo, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("D,chi,k", [(2, 8, 1), (3, 9, 2), (4, 16, 3), (6, 12, 0)])
+def test_quarter_tensor_mixed_within_fp32_error(hip, D, chi, k):
+    """Mixed precision: the T2 product on split-bf16 planes (three pieces)
+    and the double-layer absorption in FP32 on the matrix cores; every
+    element within FP32-class error of the oracle (relative to max |Q|)."""
+    A = oc.random_site_tensors(1, 1, D, 2, seed=D * 10 + k + 5)[(0, 0)]
+    rng = np.random.default_rng(chi + k + 5)
+    C = [rng.standard_normal((chi, chi)) for _ in range(4)]
+    E = [rng.standard_normal((chi, chi, D, D)) for _ in range(4)]
+    site = {'A': A, 'C': C, 'E': E}
+    Qo, _ = oc.make_quarter_tensor(site, k)
+    Qh = hip.quarter_tensor((dev(A), [dev(c) for c in C], [dev(e) for e in E]), k,
+                            precision="mixed").cpu().numpy()
+    Qh = Qh / np.abs(Qh).max()
+    err = np.abs(Qh - Qo).max()
+    # FP32 accumulation over K = chi, then over D^4 d terms: a few 1e-7
+    assert err < 5e-6, err
+    assert err > 0.0  # the mixed path really ran (FP64 would agree to 1e-13)

@@ -41,8 +41,12 @@ QuarterDims quarter_dims(const acetn_site* s, int k) {
   return q;
 }
 
-void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
+// f32 (mixed precision only): T2 and Q are held in FP32, Q as a float array
+// of the same element count (the caller's split of Q into bf16 planes reads
+// FP32 either way)
+void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q, bool f32 = false) {
   SplitScope sc(c, 3);
+  f32 = f32 && c.mixed;
   const QuarterDims qd = quarter_dims(s, k);
   const int kk = k & 3;
   const int64_t D = qd.D, D2 = D * D;
@@ -58,7 +62,14 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
   g.bk = plain(D2); g.bn = grouped((int)D2, 1, qd.ca * D2);
   g.cm = plain(qd.N2); g.cn = plain(1);
   g.alpha = 1.0; g.beta = 0.0;
-  gemm(c, g, T1, s->E[(3 + kk) & 3], T2);
+  if (f32 && c.launch) {
+    GemmBatch b{};
+    b.count = 1;
+    b.A[0] = T1; b.B[0] = s->E[(3 + kk) & 3]; b.C[0] = T2;
+    gemm_split(g, b, c.split, DType::F64, DType::F64, DType::F32, c.gwork, c.gwork_elems, c.st);
+  } else {
+    gemm(c, g, T1, s->E[(3 + kk) & 3], T2);
+  }
   if (!c.launch) return;
   int64_t ps[4];
   permuted_strides(s, kk, ps);
@@ -73,6 +84,7 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q) {
   // Y = Q viewed as [c][r][R][e][d][D]
   dl.y_o1 = D2 * qd.N2; dl.y_k3 = D * qd.N2; dl.y_b3 = qd.N2;
   dl.y_o2 = D2; dl.y_k4 = D; dl.y_b4 = 1;
+  dl.x_f32 = dl.y_f32 = f32 ? 1 : 0;
   // mixed precision: the absorption in FP32 on the matrix cores
   if (c.mixed) double_layer_f32(dl, T2, s->A, Q, c.st);
   else double_layer_f64(dl, T2, s->A, Q, c.st);
@@ -211,12 +223,13 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
 
   const bool replay = c.run && phase == 2;
   if (replay) c.launch = false;
-  quarter_tensor(c, s1, k, Q1);
-  quarter_tensor(c, s4, k + 3, Q4);
+  // mixed precision: Q1 and Q4 only feed the bf16 plane split, so they stay FP32
+  quarter_tensor(c, s1, k, Q1, true);
+  quarter_tensor(c, s4, k + 3, Q4, true);
   if (c.mixed) {
     if (c.launch) {
-      split_transpose(Q1, (int)m, (int)kk, kk, q1n, q1t, c.st);
-      split_transpose(Q4, (int)kk, (int)n, n, q4n, q4t, c.st);
+      split_transpose(Q1, (int)m, (int)kk, kk, q1n, q1t, c.st, true);
+      split_transpose(Q4, (int)kk, (int)n, n, q4n, q4t, c.st, true);
     } else if (c.run) {  // phase 2 replay: the planes are in the workspace already
       q1n.rows = (int)m; q1n.kp = (int)((kk + 31) / 32 * 32); q1n.pieces = 3;
       q1t.rows = (int)kk; q1t.kp = (int)((m + 31) / 32 * 32); q1t.pieces = 3;

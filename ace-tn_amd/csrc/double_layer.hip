@@ -96,7 +96,7 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
   // X tile of a group, element e = (row, col) of the stacked [MP][KP] tile;
   // loaded into registers one group ahead of its use.  The in-block part of
   // each element's offset is fixed, only the block base changes per group.
-  double xr[S::XPT];
+  T xr[S::XPT];
   int x_loff[S::XPT];
   int x_blk[S::XPT];
 #pragma unroll
@@ -123,7 +123,12 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
 #pragma unroll
       for (int bb = 0; bb < S::NB; ++bb)
         if (x_blk[t] == bb) bo = base[bb];
-      xr[t] = (bo >= 0) ? X[bo + x_loff[t]] : 0.0;
+      if (bo < 0)
+        xr[t] = (T)0;
+      else if (g.x_f32)
+        xr[t] = (T)reinterpret_cast<const float*>(X)[bo + x_loff[t]];
+      else
+        xr[t] = (T)X[bo + x_loff[t]];
     }
   };
   if ((int)blockIdx.x < ngroups) load_x(blockIdx.x);
@@ -205,7 +210,11 @@ __global__ __launch_bounds__(NT) void dl_mfma_kernel(DLDesc g, const double* __r
                   const int m = tm * 16 + M::row(lane, r);
                   if (m < S::D2) {
                     const int k3 = m / D, k4 = m % D;
-                    Y[yb + k3 * g.y_k3 + b3 * g.y_b3 + k4 * g.y_k4 + b4 * g.y_b4] = (double)yacc[t][r];
+                    const int64_t yo = yb + k3 * g.y_k3 + b3 * g.y_b3 + k4 * g.y_k4 + b4 * g.y_b4;
+                    if (g.y_f32)
+                      reinterpret_cast<float*>(Y)[yo] = (float)yacc[t][r];
+                    else
+                      Y[yo] = (double)yacc[t][r];
                   }
                 }
               }
@@ -246,7 +255,8 @@ void launch_dl(const DLDesc& g, const double* X, const double* A, double* Y, hip
   ACETN_HIP_CHECK(hipGetLastError());
   // algorithmic bytes: the X blocks read once, the Y blocks written once
   const double yblk = g.open_p ? (double)g.d * g.d : 1.0;
-  kernel_timing_end(e0, s, kKDoubleLayer, flops, 8.0 * (double)nblk * S::D2 * S::D2 * (1.0 + yblk));
+  const double xb = g.x_f32 ? 4.0 : 8.0, yb = g.y_f32 ? 4.0 : 8.0;
+  kernel_timing_end(e0, s, kKDoubleLayer, flops, (double)nblk * S::D2 * S::D2 * (xb + yb * yblk));
 }
 }  // namespace
 
@@ -271,6 +281,7 @@ void double_layer_t(const DLDesc& g, const double* X, const double* A, double* Y
 
 void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
                       hipStream_t s) {
+  ACETN_REQUIRE(!g.x_f32 && !g.y_f32, "double_layer_f64: FP64 X and Y only");
   double_layer_t<double>(g, X, A, Y, s);
 }
 

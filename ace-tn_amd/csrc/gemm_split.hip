@@ -38,6 +38,16 @@ constexpr int kCUs = 256;  // MI355X compute units
 
 __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; }
 
+// C = alpha v + beta C at one element, C in FP64 or (g.c_f32) FP32
+__device__ __forceinline__ void store_c(const GemmDesc& g, double* C, int64_t off, double v) {
+  if (g.c_f32) {
+    float* Cf = reinterpret_cast<float*>(C);
+    Cf[off] = (float)((g.beta != 0.0) ? g.alpha * v + g.beta * (double)Cf[off] : g.alpha * v);
+  } else {
+    C[off] = (g.beta != 0.0) ? g.alpha * v + g.beta * C[off] : g.alpha * v;
+  }
+}
+
 template <int SPLIT>
 __device__ __forceinline__ void split(float x, __bf16 (&o)[SPLIT]) {
   float r = x;
@@ -117,14 +127,16 @@ constexpr int TT = 64;
 __global__ __launch_bounds__(256) void split_transpose_kernel(const double* __restrict__ X, int R,
                                                               int Cc, int64_t ld, int Rp, int Cp,
                                                               __bf16* __restrict__ PN,
-                                                              __bf16* __restrict__ PT) {
+                                                              __bf16* __restrict__ PT, int x_f32) {
   __shared__ __bf16 tile[3][TT][TT + 8];
   const int r0 = blockIdx.y * TT, c0 = blockIdx.x * TT;
   const int64_t pn = (int64_t)R * Cp, pt = (int64_t)Cc * Rp;
   for (int e = threadIdx.x; e < TT * TT; e += 256) {
     const int rr = e / TT, cc = e % TT;
     const int r = r0 + rr, c = c0 + cc;
-    const float x = (r < R && c < Cc) ? (float)X[(int64_t)r * ld + c] : 0.f;
+    float x = 0.f;
+    if (r < R && c < Cc)
+      x = x_f32 ? reinterpret_cast<const float*>(X)[(int64_t)r * ld + c] : (float)X[(int64_t)r * ld + c];
     __bf16 pc[3];
     split<3>(x, pc);
 #pragma unroll
@@ -285,23 +297,33 @@ __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __b
     compute();
   }
 
+  // one loop per destination, the choice made once (uniform per launch)
+  auto each = [&](auto&& put) {
 #pragma unroll
-  for (int i = 0; i < WM; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-    for (int j = 0; j < WN; ++j)
+      for (int j = 0; j < WN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
-        const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
-        if (m >= g.M || n >= g.N) continue;
-        if (ksplit == 1) {
-          const double v = (double)acc[i][j][r];
-          const int64_t off = imap(g.cm, m) + imap(g.cn, n);
-          C[off] = (g.beta != 0.0) ? g.alpha * v + g.beta * C[off] : g.alpha * v;
-        } else {
-          work[((int64_t)kpart * g.M + m) * g.N + n] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
+          const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
+          if (m < g.M && n < g.N) put(m, n, acc[i][j][r]);
         }
-      }
+  };
+  if (ksplit != 1) {
+    each([&](int m, int n, float v) { work[((int64_t)kpart * g.M + m) * g.N + n] = v; });
+  } else if (g.c_f32) {
+    float* Cf = reinterpret_cast<float*>(C);
+    each([&](int m, int n, float v) {
+      const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+      Cf[off] = (float)((g.beta != 0.0) ? g.alpha * v + g.beta * (double)Cf[off] : g.alpha * v);
+    });
+  } else {
+    each([&](int m, int n, float v) {
+      const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+      C[off] = (g.beta != 0.0) ? g.alpha * (double)v + g.beta * C[off] : g.alpha * (double)v;
+    });
+  }
 }
 
 // General products (operands FP64 through their index maps, split in
@@ -429,23 +451,33 @@ __global__ __launch_bounds__(NT8) void gemm_general_kernel(GemmDesc g, const dou
     }
   }
 
+  // one loop per destination, the choice made once (uniform per launch)
+  auto each = [&](auto&& put) {
 #pragma unroll
-  for (int i = 0; i < WM; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-    for (int j = 0; j < WN; ++j)
+      for (int j = 0; j < WN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
-        const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
-        if (m >= g.M || n >= g.N) continue;
-        if (ksplit == 1) {
-          const double v = (double)acc[i][j][r];
-          const int64_t off = imap(g.cm, m) + imap(g.cn, n);
-          C[off] = (g.beta != 0.0) ? g.alpha * v + g.beta * C[off] : g.alpha * v;
-        } else {
-          work[((int64_t)kpart * g.M + m) * g.N + n] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
+          const int n = n0 + wn * 16 * WN + j * 16 + (lane & 15);
+          if (m < g.M && n < g.N) put(m, n, acc[i][j][r]);
         }
-      }
+  };
+  if (ksplit != 1) {
+    each([&](int m, int n, float v) { work[((int64_t)kpart * g.M + m) * g.N + n] = v; });
+  } else if (g.c_f32) {
+    float* Cf = reinterpret_cast<float*>(C);
+    each([&](int m, int n, float v) {
+      const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+      Cf[off] = (float)((g.beta != 0.0) ? g.alpha * v + g.beta * (double)Cf[off] : g.alpha * v);
+    });
+  } else {
+    each([&](int m, int n, float v) {
+      const int64_t off = imap(g.cm, m) + imap(g.cn, n);
+      C[off] = (g.beta != 0.0) ? g.alpha * (double)v + g.beta * C[off] : g.alpha * (double)v;
+    });
+  }
 }
 
 __global__ void splitk_reduce_f32(GemmDesc g, double* __restrict__ C, int ksplit,
@@ -467,7 +499,7 @@ __global__ void splitk_reduce_f32(GemmDesc g, double* __restrict__ C, int ksplit
     for (; p < ksplit; ++p) s += (double)work[(int64_t)p * mn + e];
     const int m = (int)(e / g.N), n = (int)(e % g.N);
     const int64_t off = imap(g.cm, m) + imap(g.cn, n);
-    C[off] = (g.beta != 0.0) ? g.alpha * s + g.beta * C[off] : g.alpha * s;
+    store_c(g, C, off, s);
   }
 }
 
@@ -671,7 +703,7 @@ void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* B
   ACETN_HIP_CHECK(hipGetLastError());
   // algorithmic bytes: both operands as SPLIT bf16 planes, C once in FP64
   kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K,
-                    2.0 * SPLIT * ((double)g.M + g.N) * Kp + 8.0 * g.M * g.N);
+                    2.0 * SPLIT * ((double)g.M + g.N) * Kp + (g.c_f32 ? 4.0 : 8.0) * g.M * g.N);
   if (ks > 1 && Cp != nullptr) {
     const dim3 grid((Cp->kp + 63) / 64, (g.N + 63) / 64);
     __bf16* out = reinterpret_cast<__bf16*>(Cp->p);
@@ -731,8 +763,14 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
                 double* work, size_t work_elems, hipStream_t s) {
   ACETN_REQUIRE(b.count == 1, "gemm_split: batched products are not supported");
   ACETN_REQUIRE(split == 2 || split == 3, "gemm_split: split must be 2 or 3");
-  ACETN_REQUIRE(ta == DType::F64 && tb == DType::F64 && tc == DType::F64,
-                "gemm_split: FP64 operands and result only");
+  ACETN_REQUIRE(ta == DType::F64 && tb == DType::F64 && (tc == DType::F64 || tc == DType::F32),
+                "gemm_split: FP64 operands, FP64 or FP32 result");
+  if (tc == DType::F32 && !g.c_f32) {
+    GemmDesc g2 = g;
+    g2.c_f32 = true;
+    gemm_split(g2, b, split, ta, tb, tc, work, work_elems, s);
+    return;
+  }
   if (g.M <= 0 || g.N <= 0) return;
   ACETN_REQUIRE(g.K > 0 || g.beta == 0.0, "gemm_split: K == 0 with beta != 0 unsupported");
   if (2.0 * g.M * g.N * (double)g.K >= kPlanarFlops) {
@@ -764,6 +802,7 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
   ACETN_REQUIRE(A.p != nullptr && A.rows == g.M && A.kp == padded_k(g.K) && A.pieces >= split,
                 "gemm_split: A planes do not match the product");
   if (g.M <= 0 || g.N <= 0) return;
+  ACETN_REQUIRE(!(g.c_f32 && Cp != nullptr), "gemm_split: C planes come from an FP64 product");
   ACETN_REQUIRE(work != nullptr && work_elems >= partial_elems(g, 1) + plane_elems(g, split),
                 "gemm_split: scratch too small");
   int ks = choose_ksplit_run(g, split);
@@ -799,13 +838,13 @@ size_t split_planes_elems(int rows, int K) {
 }
 
 void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N, SplitPlanes& T,
-                     hipStream_t s) {
+                     hipStream_t s, bool x_f32) {
   ACETN_REQUIRE(N.p && T.p, "split_transpose: null planes");
   N.rows = R; N.kp = padded_k(Cc); N.pieces = 3;
   T.rows = Cc; T.kp = padded_k(R); T.pieces = 3;
   dim3 grid((std::max(N.kp, Cc) + TT - 1) / TT, (std::max(T.kp, R) + TT - 1) / TT);
   hipLaunchKernelGGL(split_transpose_kernel, grid, dim3(256), 0, s, X, R, Cc, ld, T.kp, N.kp,
-                     reinterpret_cast<__bf16*>(N.p), reinterpret_cast<__bf16*>(T.p));
+                     reinterpret_cast<__bf16*>(N.p), reinterpret_cast<__bf16*>(T.p), x_f32 ? 1 : 0);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

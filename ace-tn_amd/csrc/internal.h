@@ -81,6 +81,7 @@ struct GemmDesc {
   IdxMap cm, cn;  // C(m,n) = C[cm(m) + cn(n)]
   double alpha, beta;
   const int* run_if = nullptr;  // device flag: when given and zero the product is skipped
+  bool c_f32 = false;           // split-bf16 products only: C holds FP32 (the pointer is a float*)
 };
 
 struct GemmBatch {
@@ -117,7 +118,7 @@ int split_planes_kp(int K);  // kp of K
 // Both orientations of X (R x Cc, row-major, leading dimension ld): N holds X
 // itself (rows R, k = columns), T holds X^T (rows Cc, k = rows of X).
 void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N, SplitPlanes& T,
-                     hipStream_t s);
+                     hipStream_t s, bool x_f32 = false);  // x_f32: X holds FP32
 // C = alpha A B + beta C with A given as planes (g.am/g.ak unused; A(m, k) =
 // the planes' row m, column k).  Bp, when given, holds B already split into
 // planes of the next product's layout (rows N, k = K; g.bk/g.bn unused).
@@ -162,11 +163,13 @@ struct DLDesc {
   int64_t a_p;
   int32_t open_p;
   int64_t y_p, y_pb;
+  int32_t x_f32 = 0, y_f32 = 0;  // FP32 path only: X / Y hold FP32 (the pointers are float*)
 };
 void double_layer_f64(const DLDesc& g, const double* X, const double* A, double* Y,
                       hipStream_t s);
 // The same contraction with FP32 operands, products and accumulation on the
-// matrix cores (v_mfma_f32_16x16x4f32), FP64 in and out: the mixed-precision path.
+// matrix cores (v_mfma_f32_16x16x4f32): the mixed-precision path.  X and Y
+// are FP64 unless g.x_f32 / g.y_f32 say FP32.
 void double_layer_f32(const DLDesc& g, const double* X, const double* A, double* Y,
                       hipStream_t s);
 

@@ -95,8 +95,13 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q, bool f32 = fa
 // (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020) with rank guard.
 // Result is written back into Y.
 // ---------------------------------------------------------------------------
-void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
+// Returns the buffer that holds the basis: Y, or, for a basis that only steers
+// the power iteration in mixed precision (steer), the output of the single
+// shifted CholeskyQR pass (well conditioned, not orthonormal to working
+// precision: the next orthonormalisation restores that).
+const double* orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q, bool steer = false) {
   SplitScope sc(c, 0);
+  const int passes = (steer && c.mixed) ? 1 : 4;
   double* G = c.take((size_t)q * q);
   double* Rinv = c.take((size_t)q * q);
   double* Yt = c.take((size_t)m * q);
@@ -107,7 +112,7 @@ void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
   // pass 0: shifted (Y -> Yt); pass 1: plain with rank guard (Yt -> Y), dead
   // columns refilled; passes 2, 3 (Y -> Yt -> Y) orthogonalise the refilled
   // columns and are skipped on the device when no column died.
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     double* src = (pass % 2 == 0) ? Y : Yt;
     double* dst = (pass % 2 == 0) ? Yt : Y;
     const int* run_if = (pass >= 2 && c.launch) ? dead + q : nullptr;
@@ -122,7 +127,8 @@ void orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q) {
     gemm(c, gr, src, Rinv, dst);
     if (pass == 1 && c.launch) refill_dead_columns(dst, m, (int)q, dead, 0x243F6A8885A308D3ull, c.st);
   }
-  // the result is in Y whether or not passes 2, 3 ran
+  // with four passes the result is in Y whether or not passes 2, 3 ran
+  return passes == 1 ? Yt : Y;
 }
 
 // ---------------------------------------------------------------------------
@@ -248,8 +254,8 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q, &wp);
   for (int it = 0; it < niter; ++it) {
     if (it >= 1) c.split = c.mixed ? 3 : 0;
-    orthonormalize(c, Y, m, q);
-    mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Y, q, W, q, nullptr, &wp);  // Q1^H Y
+    const double* Yb = orthonormalize(c, Y, m, q, true);
+    mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Yb, q, W, q, nullptr, &wp);  // Q1^H Y
     mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, W, q, Z, q, &wp, &zp);       // Q4^H (.)
     mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Z, q, W, q, &zp, &wp);      // Q4 (.)
     mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q, &wp);          // Q1 (.)

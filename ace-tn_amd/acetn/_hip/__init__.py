@@ -422,8 +422,9 @@ class ProjectorJob:
         self.ws = slot_workspace(slot if ws_key is None else ("pair", ws_key), nb, self.dev)
         self.cd_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
         nspec = max(chi + oversampling, 512) if self.svd else chi + oversampling
-        self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
         with torch.cuda.stream(self.stream):
+            # zero-filled on the pair's stream, ahead of the kernels that write it
+            self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
             check(lib().acetn_hip_projectors_begin(
                 self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
                 self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec), ptr(self.ws),

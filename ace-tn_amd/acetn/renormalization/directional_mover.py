@@ -141,10 +141,20 @@ def run_stages(items):
             m.stage(net, mv)
         return
     multi = len(items) > 1
-    plans, base = [], 0
+    plans, base, joined = [], 0, set()
     for mover, ipeps, moves in items:
         jobs, targets = mover._plan(ipeps, moves)
         calc = mover.projector_calculator
+        if jobs:
+            # the side streams first catch up with the caller's stream, where
+            # the boundary tensors the pairs read may still be in flight
+            dev = ipeps[tuple(jobs[0][0][0])]['A'].device
+            if str(dev) not in joined:
+                joined.add(str(dev))
+                ev = torch.cuda.Event()
+                ev.record(torch.cuda.current_stream(dev))
+                for sl in range(1, _hip.N_SLOTS):
+                    _hip.slot_stream(sl, dev).wait_event(ev)
         started = calc.start_many(ipeps, jobs, base=base if multi else None)
         plans.append({"mover": mover, "ipeps": ipeps, "targets": targets, "started": started,
                       "base": base, "proj": [None] * len(jobs), "done": [None] * len(jobs),

@@ -246,14 +246,14 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
     q1n.p = q1t.p = q4n.p = q4t.p = nullptr;
   }
   if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st);
-  // mixed precision: the sketch and the first power iteration only steer the
-  // range finder (bf16x3); later products carry the basis itself (bf16x6)
+  // mixed precision: the sketch and the power iterations only steer the range
+  // finder (bf16x3); the products after the final orthonormalisation carry the
+  // basis itself (bf16x6)
   SplitScope sc(c, 2);
   // Y = Q1 (Q4 Omega)
   mm_corner(c, q4n, (int)kk, (int)q, (int)n, false, Q4, n, Om, q, W, q, nullptr, &wp);
   mm_corner(c, q1n, (int)m, (int)q, (int)kk, false, Q1, kk, W, q, Y, q, &wp);
   for (int it = 0; it < niter; ++it) {
-    if (it >= 1) c.split = c.mixed ? 3 : 0;
     const double* Yb = orthonormalize(c, Y, m, q, true);
     mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Yb, q, W, q, nullptr, &wp);  // Q1^H Y
     mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, W, q, Z, q, &wp, &zp);       // Q4^H (.)

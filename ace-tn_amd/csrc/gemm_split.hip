@@ -620,7 +620,9 @@ struct Tile {
 };
 Tile choose_tile(const GemmDesc& g) {
   if (g.N > 96 && g.N <= 160) return g.M >= 2048 ? Tile{2, 5} : Tile{1, 5};
-  if (g.M >= 1024 && g.N >= 1024) return Tile{4, 4};
+  // short k loops (the quarter tensor's K = chi) finish a tile in a few steps:
+  // smaller tiles keep more of them in flight
+  if (g.M >= 1024 && g.N >= 1024) return g.K <= 256 ? Tile{2, 2} : Tile{4, 4};
   return Tile{2, 2};
 }
 

@@ -47,9 +47,13 @@ typedef struct acetn_site {
  *   ACETN_HIP_FP64:  FP64 matrix cores throughout (the plain method).
  *   ACETN_HIP_MIXED: enlarged-corner, range-finder and renormalisation
  *     products on the bf16 matrix cores with FP32 accumulation, operands split
- *     into bf16 pieces (bf16x3 for the sketch and the first power iteration,
- *     bf16x6 for the rest); the double-layer absorption, the QR, the projector
- *     SVD and the normalisations stay FP64.  Storage stays FP64.
+ *     into bf16 pieces (bf16x3 for the sketch and the power iterations, bf16x6
+ *     after the final orthonormalisation); the double-layer absorption on the
+ *     FP32 matrix cores; the bases of the power iterations take one shifted
+ *     CholeskyQR pass.  The QR of the final basis, the projector SVD and the
+ *     normalisations stay FP64, and so does every tensor that crosses this
+ *     interface (the half-system corners are held in FP32 inside the
+ *     workspace).
  * No reference counterpart: the reference runs one dtype end to end
  * (acetn/ipeps/ipeps_config.py:15, ``dtype``). */
 enum { ACETN_HIP_FP64 = 0, ACETN_HIP_MIXED = 1 };

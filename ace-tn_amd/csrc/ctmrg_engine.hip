@@ -540,10 +540,13 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     const int64_t xa = s1->E_dim[ke][0], xb = s1->E_dim[ke][1];
     double* T = c.take((size_t)xa * D2 * D2 * cDj);
     double* Zt = c.take((size_t)cDj * D2 * xa * D2);
+    // T[a][(u,U)][j][(l,L)]: the columns run (j, (l,L)), so each block of the
+    // double layer below, (a, j), holds its (l,L) pairs contiguously and the
+    // blocks of a row follow each other (coalesced X loads)
     GemmDesc g{};
     g.M = (int)(xa * D2); g.N = (int)(D2 * cDj); g.K = (int)xb;
     g.am = grouped((int)D2, 1, xb * D2); g.ak = plain(D2);
-    g.bk = plain(D2 * cDj); g.bn = plain(1);
+    g.bk = plain(D2 * cDj); g.bn = grouped((int)D2, cDj, 1);
     g.cm = plain(D2 * cDj); g.cn = plain(1);
     g.alpha = 1.0; g.beta = 0.0;
     gemm(c, g, s1->E[ke], P1j, T);
@@ -554,7 +557,7 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
       dl.D = (int)D; dl.d = (int)s1->d; dl.n1 = (int)xa; dl.n2 = (int)cDj;
       const int64_t ld = D2 * cDj;  // row length of T
       dl.x_o1 = D2 * ld; dl.x_k1 = D * ld; dl.x_b1 = ld;
-      dl.x_k2 = D * cDj; dl.x_b2 = cDj; dl.x_o2 = 1;
+      dl.x_o2 = D2; dl.x_k2 = D; dl.x_b2 = 1;
       dl.a_leg[0] = ps[0]; dl.a_leg[1] = ps[1]; dl.a_leg[2] = ps[2]; dl.a_leg[3] = ps[3];
       dl.a_p = 1;
       // Z[x][r][R][a][d][D]

@@ -549,7 +549,24 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     g.bk = plain(D2 * cDj); g.bn = grouped((int)D2, cDj, 1);
     g.cm = plain(D2 * cDj); g.cn = plain(1);
     g.alpha = 1.0; g.beta = 0.0;
-    gemm(c, g, s1->E[ke], P1j, T);
+    // mixed precision: T only feeds the FP32 double layer, so it is stored in
+    // FP32, and so is Z when the last product splits it into planes
+    GemmDesc h{};
+    h.M = (int)(cDj * D2); h.N = (int)cDi; h.K = (int)(xa * D2);
+    h.am = plain(xa * D2); h.ak = plain(1);
+    h.bk = plain(cDi); h.bn = plain(1);
+    h.cm = plain(1); h.cn = plain(cDj * D2);
+    h.alpha = 1.0; h.beta = 0.0;
+    const bool t32 = c.mixed, z32 = c.mixed && gemm_split_is_planar(h);
+    h.a_f32 = z32;
+    if (t32 && c.launch) {
+      GemmBatch b{};
+      b.count = 1;
+      b.A[0] = s1->E[ke]; b.B[0] = P1j; b.C[0] = T;
+      gemm_split(g, b, c.split, DType::F64, DType::F64, DType::F32, c.gwork, c.gwork_elems, c.st);
+    } else {
+      gemm(c, g, s1->E[ke], P1j, T);
+    }
     if (c.launch) {
       int64_t ps[4];
       permuted_strides(s1, k, ps);
@@ -564,15 +581,11 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
       const int64_t rowz = xa * D2;
       dl.y_o2 = D2 * rowz; dl.y_k3 = D * rowz; dl.y_b3 = rowz;
       dl.y_o1 = D2; dl.y_k4 = D; dl.y_b4 = 1;
+      dl.x_f32 = t32 ? 1 : 0;
+      dl.y_f32 = z32 ? 1 : 0;
       if (c.mixed) double_layer_f32(dl, T, s1->A, Zt, c.st);
       else double_layer_f64(dl, T, s1->A, Zt, c.st);
     }
-    GemmDesc h{};
-    h.M = (int)(cDj * D2); h.N = (int)cDi; h.K = (int)(xa * D2);
-    h.am = plain(xa * D2); h.ak = plain(1);
-    h.bk = plain(cDi); h.bn = plain(1);
-    h.cm = plain(1); h.cn = plain(cDj * D2);
-    h.alpha = 1.0; h.beta = 0.0;
     gemm(c, h, Zt, P2i, Eo);
     if (c.launch) frobenius_normalize(Eo, cDi * cDj * D2, nrm, c.st);
   }

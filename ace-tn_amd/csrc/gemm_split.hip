@@ -61,12 +61,16 @@ __device__ __forceinline__ void split(float x, __bf16 (&o)[SPLIT]) {
 // planes[s][n][k] (k padded to Kp) of B(k, n) = B[bk(k) + bn(n)]
 template <int SPLIT>
 __global__ void split_planes_kernel(const double* __restrict__ B, IdxMap bk, IdxMap bn, int K,
-                                    int N, int Kp, __bf16* __restrict__ planes) {
+                                    int N, int Kp, __bf16* __restrict__ planes, int b_f32) {
   const int64_t total = (int64_t)N * Kp;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int n = (int)(e / Kp), k = (int)(e % Kp);
-    const float x = (k < K) ? (float)B[imap(bk, k) + imap(bn, n)] : 0.f;
+    float x = 0.f;
+    if (k < K) {
+      const int64_t o = imap(bk, k) + imap(bn, n);
+      x = b_f32 ? reinterpret_cast<const float*>(B)[o] : (float)B[o];
+    }
     __bf16 pc[SPLIT];
     split<SPLIT>(x, pc);
 #pragma unroll
@@ -79,14 +83,19 @@ __global__ void split_planes_kernel(const double* __restrict__ B, IdxMap bk, Idx
 template <int SPLIT>
 __global__ __launch_bounds__(256) void split_planes_t_kernel(const double* __restrict__ B, IdxMap bk,
                                                              IdxMap bn, int K, int N, int Kp,
-                                                             __bf16* __restrict__ planes) {
+                                                             __bf16* __restrict__ planes, int b_f32) {
   __shared__ float tile[64][65];
   const int k0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
   const int64_t total = (int64_t)N * Kp;
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
     const int kk = e / 64, nn = e % 64;
     const int k = k0 + kk, n = n0 + nn;
-    tile[kk][nn] = (k < K && n < N) ? (float)B[imap(bk, k) + imap(bn, n)] : 0.f;
+    float x = 0.f;
+    if (k < K && n < N) {
+      const int64_t o = imap(bk, k) + imap(bn, n);
+      x = b_f32 ? reinterpret_cast<const float*>(B)[o] : (float)B[o];
+    }
+    tile[kk][nn] = x;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
@@ -105,17 +114,17 @@ __global__ __launch_bounds__(256) void split_planes_t_kernel(const double* __res
 // index is unit-stride
 template <int SPLIT>
 void split_into_planes(const double* X, const IdxMap& km, const IdxMap& nm, int K, int N, int Kp,
-                       __bf16* planes, hipStream_t s) {
+                       __bf16* planes, hipStream_t s, bool x_f32 = false) {
   if (nm.s0 == 1 && km.s0 != 1) {
     dim3 grid((Kp + 63) / 64, (N + 63) / 64);
     ACETN_REQUIRE(grid.y <= 65535, "split_planes: too many rows");
     hipLaunchKernelGGL(split_planes_t_kernel<SPLIT>, grid, dim3(256), 0, s, X, km, nm, K, N, Kp,
-                       planes);
+                       planes, x_f32 ? 1 : 0);
   } else {
     const int64_t total = (int64_t)N * Kp;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(split_planes_kernel<SPLIT>, dim3(blocks), dim3(256), 0, s, X, km, nm, K, N,
-                       Kp, planes);
+                       Kp, planes, x_f32 ? 1 : 0);
   }
   ACETN_HIP_CHECK(hipGetLastError());
 }
@@ -775,7 +784,7 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
   }
   if (g.M <= 0 || g.N <= 0) return;
   ACETN_REQUIRE(g.K > 0 || g.beta == 0.0, "gemm_split: K == 0 with beta != 0 unsupported");
-  if (2.0 * g.M * g.N * (double)g.K >= kPlanarFlops) {
+  if (gemm_split_is_planar(g)) {
     const size_t apl = split_planes_elems(g.M, g.K);
     int ks = choose_ksplit_run(g, split);
     if (work_elems < partial_elems(g, ks) + plane_elems(g, 3) + apl) ks = 1;
@@ -785,12 +794,13 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
     __bf16* Ap = reinterpret_cast<__bf16*>(work + partial_elems(g, ks) + plane_elems(g, 3));
     const int64_t total = (int64_t)g.M * Kp;
     // planes[s][m][k] of A(m, k) = A[am(m) + ak(k)]: the B-plane split with the maps swapped
-    if (split == 2) split_into_planes<2>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s);
-    else split_into_planes<3>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s);
+    if (split == 2) split_into_planes<2>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s, g.a_f32);
+    else split_into_planes<3>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s, g.a_f32);
     if (split == 2) run<2>(g, Ap, total, b.B[0], b.C[0], ks, work, s);
     else run<3>(g, Ap, total, b.B[0], b.C[0], ks, work, s);
     return;
   }
+  ACETN_REQUIRE(!g.a_f32, "gemm_split: an FP32 A operand needs the plane path");
   int ks = general_ksplit(g);
   if (work == nullptr || work_elems < general_partial_elems(g, ks)) ks = 1;
   if (split == 2) run_general<2>(g, b.A[0], b.B[0], b.C[0], ks, work, s);
@@ -833,6 +843,8 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
 }
 
 int split_planes_kp(int K) { return padded_k(K); }
+
+bool gemm_split_is_planar(const GemmDesc& g) { return 2.0 * g.M * g.N * (double)g.K >= kPlanarFlops; }
 
 size_t split_planes_elems(int rows, int K) {
   const size_t bytes = (size_t)3 * rows * padded_k(K) * sizeof(__bf16);

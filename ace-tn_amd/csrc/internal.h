@@ -82,6 +82,7 @@ struct GemmDesc {
   double alpha, beta;
   const int* run_if = nullptr;  // device flag: when given and zero the product is skipped
   bool c_f32 = false;           // split-bf16 products only: C holds FP32 (the pointer is a float*)
+  bool a_f32 = false;           // split-bf16 products on the plane path only: A holds FP32
 };
 
 struct GemmBatch {
@@ -115,6 +116,8 @@ struct SplitPlanes {
 };
 size_t split_planes_elems(int rows, int K);
 int split_planes_kp(int K);  // kp of K
+// whether gemm_split runs this product from bf16 planes (the path that accepts a_f32)
+bool gemm_split_is_planar(const GemmDesc& g);
 // Both orientations of X (R x Cc, row-major, leading dimension ld): N holds X
 // itself (rows R, k = columns), T holds X^T (rows Cc, k = rows of X).
 void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N, SplitPlanes& T,

@@ -78,6 +78,13 @@ def _declare(lib):
                                                           c_vp, c_vp, c_i64, c_vp, c_vp, c_i64,
                                                           ctypes.c_int, c_vp, c_vp, c_vp, c_vp,
                                                           c_size, c_vp]),
+        "acetn_hip_renormalize_part_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                          ctypes.c_int, c_i64, c_i64]),
+        "acetn_hip_renormalize_boundary_part": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
+                                                               ctypes.c_int, ctypes.c_int, c_vp, c_vp,
+                                                               c_i64, c_vp, c_vp, c_i64, c_i64,
+                                                               ctypes.c_int, c_vp, c_vp, c_vp,
+                                                               c_vp, c_size, c_vp]),
         "acetn_hip_site_rdm_workspace": (c_size, [ctypes.POINTER(AcetnSite)]),
         "acetn_hip_site_rdm": (ctypes.c_int, [ctypes.POINTER(AcetnSite), c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_bond_rdm_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.POINTER(AcetnSite),
@@ -470,6 +477,68 @@ def renormalize_boundary(site1, k, P1i, P2i, P1j, P2j, slot=None, precision="fp6
                                                ptr(Ca), ptr(Cb), ptr(Eo),
                                                ptr(ws), ws.numel(), stream_handle()))
     return Ca, Cb, Eo
+
+
+class RenormalizeParts:
+    """renormalize_boundary in two calls on the current stream: ``first(P1, P2)``
+    with the projectors of the pair named by ``first_pair`` ('j' or 'i'),
+    then ``second(P1, P2)`` with the other pair's; ``second`` returns
+    (C_a, C_b, E_o).  ``cd_bound`` >= the second pair's count (chi) fixes the
+    shared workspace layout."""
+
+    def __init__(self, site1, k, cd_bound, first_pair="j", slot=None, precision="fp64",
+                 ws_key=None):
+        if first_pair not in ("i", "j"):
+            raise ValueError("first_pair must be 'i' or 'j'")
+        self.s1, self.keep = make_site(*site1)
+        self.k, self.bound = k, int(cd_bound)
+        self.first_code = 0 if first_pair == "j" else 1
+        self.prec = precision_code(precision)
+        self.key = slot if ws_key is None else ws_key
+        self.dev = self.keep[0].device
+
+    def _call(self, part, P1i, P2i, cDi, P1j, P2j, cDj, Ca, Cb, Eo):
+        check(lib().acetn_hip_renormalize_boundary_part(
+            ctypes.byref(self.s1), self.k, part, self.first_code, ptr(P1i), ptr(P2i), cDi,
+            ptr(P1j), ptr(P2j), cDj, self.bound, self.prec, ptr(Ca), ptr(Cb), ptr(Eo),
+            ptr(self.ws), self.ws.numel(), stream_handle()))
+
+    def first(self, P1, P2):
+        s1, k = self.s1, self.k
+        self.P1f, self.P2f = P1.contiguous(), P2.contiguous()
+        cf = self.P1f.shape[1]
+        cDi, cDj = (self.bound, cf) if self.first_code == 0 else (cf, self.bound)
+        nb = lib().acetn_hip_renormalize_part_workspace(ctypes.byref(s1), k, self.first_code,
+                                                        cDi, cDj)
+        self.ws = (workspace(nb, self.dev) if self.key is None
+                   else slot_workspace(("renormalize", self.key), nb, self.dev))
+        if self.first_code == 0:
+            xc = s1.E_dim[k % 4][1]
+            self.Cf = torch.empty(cf, xc, dtype=torch.float64, device=self.dev)
+            self._call(1, None, None, 0, self.P1f, self.P2f, cf, None, self.Cf, None)
+        else:
+            xa = s1.E_dim[(2 + k) % 4][0]
+            self.Cf = torch.empty(xa, cf, dtype=torch.float64, device=self.dev)
+            self._call(1, self.P1f, self.P2f, cf, None, None, 0, self.Cf, None, None)
+        return self.Cf
+
+    def second(self, P1, P2):
+        s1, k, D = self.s1, self.k, self.s1.D
+        P1, P2 = P1.contiguous(), P2.contiguous()
+        cs = P1.shape[1]
+        if cs > self.bound:
+            raise ValueError("renormalize: the second pair's count exceeds the layout bound")
+        if self.first_code == 0:
+            P1i, P2i, P1j, P2j, cDi, cDj = P1, P2, self.P1f, self.P2f, cs, self.P1f.shape[1]
+            Cb = self.Cf
+            Ca = torch.empty(s1.E_dim[(2 + k) % 4][0], cDi, dtype=torch.float64, device=self.dev)
+        else:
+            P1i, P2i, P1j, P2j, cDi, cDj = self.P1f, self.P2f, P1, P2, self.P1f.shape[1], cs
+            Ca = self.Cf
+            Cb = torch.empty(cDj, s1.E_dim[k % 4][1], dtype=torch.float64, device=self.dev)
+        Eo = torch.empty(cDi, cDj, D, D, dtype=torch.float64, device=self.dev)
+        self._call(2, P1i, P2i, cDi, P1j, P2j, cDj, Ca, Cb, Eo)
+        return Ca, Cb, Eo
 
 
 def site_rdm(site):

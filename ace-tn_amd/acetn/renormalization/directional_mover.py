@@ -128,8 +128,10 @@ def run_stages(items):
     pairs of a multi-network round on slot g % N_SLOTS in round order g, each
     with a workspace of its own); the host then reads each pair's truncation
     count as soon as that pair has produced it, enqueues its projector
-    products, and starts a renormalisation as soon as both of its pairs are
-    finished (its slot stream waits on their completion events).  One
+    products, and starts each renormalisation in two parts: the part that
+    the first finished of its two pairs feeds on its own at once, the rest
+    once the other pair is finished too (its slot stream waits on their
+    completion events).  One
     network's Jacobi eigensolves thereby overlap the other pairs' matrix-core
     work.  The new boundary tensors are written back after everything has
     finished.  With full-system projectors the two moves of a stage depend on
@@ -164,20 +166,34 @@ def run_stages(items):
         mover, ipeps = pl["mover"], pl["ipeps"]
         calc = mover.projector_calculator
         proj, done, results = pl["proj"], pl["done"], pl["results"]
+        parts = [None] * len(pl["targets"])
+        flat = lambda P: P.reshape(-1, P.shape[-1])
         for idx, pr, ev in calc.finish_each(ipeps, pl["started"]):
             proj[idx], done[idx] = pr, ev
             for t, (s1, s2, i, j, k) in enumerate(pl["targets"]):
-                if results[t] is not None or proj[i] is None or proj[j] is None:
+                if results[t] is not None:
                     continue
                 g = pl["base"] + t
                 slot = g % _hip.N_SLOTS if multi else t
                 stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
-                stream.wait_event(done[i])
-                stream.wait_event(done[j])
-                with torch.cuda.stream(stream):
-                    results[t] = mover._renormalize(ipeps, proj[i], proj[j], s1, k, slot,
-                                                    calc.precision,
-                                                    ws_key=g if multi else None)
+                # the part that the first finished of the two pairs feeds on
+                # its own starts at once, the rest when the other pair is done
+                # too (same stream, same workspace)
+                if parts[t] is None and (proj[j] is not None or proj[i] is not None):
+                    a = j if proj[j] is not None else i
+                    stream.wait_event(done[a])
+                    with torch.cuda.stream(stream):
+                        st = ipeps[s1]
+                        parts[t] = (_hip.RenormalizeParts(
+                            (st['A'], st['C'], st['E']), k, ipeps.dims['chi'],
+                            first_pair="j" if a == j else "i", slot=slot,
+                            precision=calc.precision, ws_key=g if multi else None), a)
+                        parts[t][0].first(flat(proj[a][0]), flat(proj[a][1]))
+                if parts[t] is not None and proj[i] is not None and proj[j] is not None:
+                    b = i if parts[t][1] == j else j
+                    stream.wait_event(done[b])
+                    with torch.cuda.stream(stream):
+                        results[t] = parts[t][0].second(flat(proj[b][0]), flat(proj[b][1]))
     torch.cuda.synchronize()
     for pl in plans:
         for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(pl["targets"], pl["results"]):

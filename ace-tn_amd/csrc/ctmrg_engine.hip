@@ -487,14 +487,27 @@ void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, 
 // ---------------------------------------------------------------------------
 // renormalize_boundary (directional_mover.py:273-366)
 // ---------------------------------------------------------------------------
+// part 0: everything in one call.  Parts 1 and 2 split it by projector pair:
+// part 1 is what the pair named by ``first`` feeds on its own (0: pair j --
+// cj2 and ej up to its double layer; 1: pair i -- cj1 and ej up to its double
+// layer, ej then contracted with P2_i first), part 2 the rest.  Both parts
+// carve the same workspace, sized with cD_layout (>= the other pair's count)
+// for the count not yet known at part 1, and part 2 finds ej's intermediate
+// where part 1 left it.
 void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const double* P2i,
                  int64_t cDi, const double* P1j, const double* P2j, int64_t cDj, double* Ca,
-                 double* Cb, double* Eo) {
+                 double* Cb, double* Eo, int part = 0, int64_t cD_layout = 0, int first = 0) {
   SplitScope sc(c, 3);
   k &= 3;
   const int64_t D = s1->D, D2 = D * D;
+  if (!c.run && cD_layout > 0) (first == 0 ? cDi : cDj) = cD_layout;
+  const bool launch = c.launch;
+  // whether a block that depends on pair i (dep_i) or on pair j runs in this call
+  auto runs = [&](bool dep_i) { return launch && (part == 0 || (part == 1) == (dep_i == (first == 1))); };
+  const int route = part == 0 ? 0 : first;
   double* nrm = c.take(1024);
   // ---- cj1: C[(3+k)] with E[(2+k)] and P1_i
+  c.launch = runs(true);
   {
     const int ke = (2 + k) & 3, kc = (3 + k) & 3;
     const int64_t xa = s1->E_dim[ke][0], xb = s1->E_dim[ke][1];
@@ -518,6 +531,7 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     if (c.launch) frobenius_normalize(Ca, xa * cDi, nrm, c.st);
   }
   // ---- cj2: C[k] with E[k] and P2_j
+  c.launch = runs(false);
   {
     const int64_t xa = s1->C_dim[k][0], xb = s1->C_dim[k][1];
     ACETN_REQUIRE(s1->E_dim[k][0] == xb, "cj2: C/E dimension mismatch");
@@ -534,51 +548,68 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     gemm(c, h, P2j, T, Cb);
     if (c.launch) frobenius_normalize(Cb, cDj * xc, nrm, c.st);
   }
-  // ---- ej: E[(3+k)] with A_k, P1_j, P2_i
+  // ---- ej: E[(3+k)] with A_k, P1_j, P2_i.  Route 0 (the reference's order)
+  // contracts E with P1_j first, route 1 with P2_i first; the double layer
+  // runs on that first intermediate and the other projector comes last.
+  c.launch = launch && part != 2;
   {
     const int ke = (3 + k) & 3;
     const int64_t xa = s1->E_dim[ke][0], xb = s1->E_dim[ke][1];
-    double* T = c.take((size_t)xa * D2 * D2 * cDj);
-    double* Zt = c.take((size_t)cDj * D2 * xa * D2);
-    // T[a][(u,U)][j][(l,L)]: the columns run (j, (l,L)), so each block of the
-    // double layer below, (a, j), holds its (l,L) pairs contiguously and the
-    // blocks of a row follow each other (coalesced X loads)
+    const bool r1 = route == 1;
+    const int64_t xf = r1 ? xb : xa;    // E index that stays through the double layer
+    const int64_t xs = r1 ? xa : xb;    // E index the first projector contracts
+    const int64_t cf = r1 ? cDi : cDj;  // columns of the first projector
+    const int64_t cs = r1 ? cDj : cDi;  // columns of the last
+    const double* Pf = r1 ? P2i : P1j;
+    const double* Ps = r1 ? P1j : P2i;
+    double* T = c.take((size_t)xf * D2 * D2 * cf);
+    double* Zt = c.take((size_t)cf * D2 * xf * D2);
+    // T[f][(e,E)][c][(p,P)] (E's own D legs (e,E), the first projector's
+    // (p,P)): the columns run (c, (p,P)), so each block of the double layer
+    // below, (f, c), holds its (p,P) pairs contiguously and the blocks of a
+    // row follow each other (coalesced X loads)
     GemmDesc g{};
-    g.M = (int)(xa * D2); g.N = (int)(D2 * cDj); g.K = (int)xb;
-    g.am = grouped((int)D2, 1, xb * D2); g.ak = plain(D2);
-    g.bk = plain(D2 * cDj); g.bn = grouped((int)D2, cDj, 1);
-    g.cm = plain(D2 * cDj); g.cn = plain(1);
+    g.M = (int)(xf * D2); g.N = (int)(D2 * cf); g.K = (int)xs;
+    g.am = r1 ? plain(1) : grouped((int)D2, 1, xb * D2);
+    g.ak = r1 ? plain(xb * D2) : plain(D2);
+    g.bk = plain(D2 * cf); g.bn = grouped((int)D2, cf, 1);
+    g.cm = plain(D2 * cf); g.cn = plain(1);
     g.alpha = 1.0; g.beta = 0.0;
+    // E_o[i][j][r][R] = Z[(c,r,R)][(f,q,Q)] . Ps[(f,q,Q)][last column]
+    GemmDesc h{};
+    h.M = (int)(cf * D2); h.N = (int)cs; h.K = (int)(xf * D2);
+    h.am = plain(xf * D2); h.ak = plain(1);
+    h.bk = plain(cs); h.bn = plain(1);
+    h.cm = r1 ? grouped((int)D2, 1, cDj * D2) : plain(1);
+    h.cn = r1 ? plain(D2) : plain(cDj * D2);
+    h.alpha = 1.0; h.beta = 0.0;
     // mixed precision: T only feeds the FP32 double layer, so it is stored in
     // FP32, and so is Z when the last product splits it into planes
-    GemmDesc h{};
-    h.M = (int)(cDj * D2); h.N = (int)cDi; h.K = (int)(xa * D2);
-    h.am = plain(xa * D2); h.ak = plain(1);
-    h.bk = plain(cDi); h.bn = plain(1);
-    h.cm = plain(1); h.cn = plain(cDj * D2);
-    h.alpha = 1.0; h.beta = 0.0;
     const bool t32 = c.mixed, z32 = c.mixed && gemm_split_is_planar(h);
     h.a_f32 = z32;
     if (t32 && c.launch) {
       GemmBatch b{};
       b.count = 1;
-      b.A[0] = s1->E[ke]; b.B[0] = P1j; b.C[0] = T;
+      b.A[0] = s1->E[ke]; b.B[0] = Pf; b.C[0] = T;
       gemm_split(g, b, c.split, DType::F64, DType::F64, DType::F32, c.gwork, c.gwork_elems, c.st);
     } else {
-      gemm(c, g, s1->E[ke], P1j, T);
+      gemm(c, g, s1->E[ke], Pf, T);
     }
     if (c.launch) {
       int64_t ps[4];
       permuted_strides(s1, k, ps);
       DLDesc dl{};
-      dl.D = (int)D; dl.d = (int)s1->d; dl.n1 = (int)xa; dl.n2 = (int)cDj;
-      const int64_t ld = D2 * cDj;  // row length of T
+      dl.D = (int)D; dl.d = (int)s1->d; dl.n1 = (int)xf; dl.n2 = (int)cf;
+      const int64_t ld = D2 * cf;  // row length of T
       dl.x_o1 = D2 * ld; dl.x_k1 = D * ld; dl.x_b1 = ld;
       dl.x_o2 = D2; dl.x_k2 = D; dl.x_b2 = 1;
-      dl.a_leg[0] = ps[0]; dl.a_leg[1] = ps[1]; dl.a_leg[2] = ps[2]; dl.a_leg[3] = ps[3];
+      // A legs: E's D leg (axis 0), the first projector's (axis 1 for P1_j,
+      // axis 3 for P2_i), the open r (axis 2), the last projector's
+      dl.a_leg[0] = ps[0]; dl.a_leg[1] = r1 ? ps[3] : ps[1];
+      dl.a_leg[2] = ps[2]; dl.a_leg[3] = r1 ? ps[1] : ps[3];
       dl.a_p = 1;
-      // Z[x][r][R][a][d][D]
-      const int64_t rowz = xa * D2;
+      // Z[c][r][R][f][q][Q]
+      const int64_t rowz = xf * D2;
       dl.y_o2 = D2 * rowz; dl.y_k3 = D * rowz; dl.y_b3 = rowz;
       dl.y_o1 = D2; dl.y_k4 = D; dl.y_b4 = 1;
       dl.x_f32 = t32 ? 1 : 0;
@@ -586,9 +617,11 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
       if (c.mixed) double_layer_f32(dl, T, s1->A, Zt, c.st);
       else double_layer_f64(dl, T, s1->A, Zt, c.st);
     }
-    gemm(c, h, Zt, P2i, Eo);
+    c.launch = launch && part != 1;
+    gemm(c, h, Zt, Ps, Eo);
     if (c.launch) frobenius_normalize(Eo, cDi * cDj * D2, nrm, c.st);
   }
+  c.launch = launch;
 }
 
 // ---------------------------------------------------------------------------
@@ -994,6 +1027,53 @@ int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       c.mixed = precision == ACETN_HIP_MIXED;
       renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o);
+    });
+  });
+}
+
+size_t acetn_hip_renormalize_part_workspace(const acetn_site* s1, int k, int first, int64_t cD_i,
+                                            int64_t cD_j) {
+  size_t r = 0;
+  guarded([&] {
+    check_site(s1);
+    ACETN_REQUIRE(first == 0 || first == 1, "renormalize_part: first must be 0 or 1");
+    r = measure([&](Ctx& c) {
+      renormalize(c, s1, k, nullptr, nullptr, cD_i, nullptr, nullptr, cD_j, nullptr, nullptr,
+                  nullptr, 1, first == 0 ? cD_i : cD_j, first);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_renormalize_boundary_part(const acetn_site* s1, int k, int part, int first,
+                                        const double* P1_i, const double* P2_i, int64_t cD_i,
+                                        const double* P1_j, const double* P2_j, int64_t cD_j,
+                                        int64_t cD_layout, int precision, double* C_a,
+                                        double* C_b, double* E_o, void* ws, size_t ws_bytes,
+                                        void* stream) {
+  return guarded([&] {
+    check_site(s1);
+    check_precision(precision);
+    ACETN_REQUIRE(part == 1 || part == 2, "renormalize_part: part must be 1 or 2");
+    ACETN_REQUIRE(first == 0 || first == 1, "renormalize_part: first must be 0 or 1");
+    ACETN_REQUIRE(cD_layout >= 1, "renormalize_part: empty projector");
+    // the pair of part 1 and the one whose count is only bounded there
+    int64_t& later = first == 0 ? cD_i : cD_j;
+    const int64_t known = first == 0 ? cD_j : cD_i;
+    ACETN_REQUIRE(known >= 1, "renormalize_part: empty projector");
+    const bool ok1 = first == 0 ? (P1_j && P2_j && C_b) : (P1_i && P2_i && C_a);
+    const bool ok2 = P1_i && P2_i && P1_j && P2_j && E_o && (first == 0 ? C_a : C_b);
+    if (part == 1) {
+      ACETN_REQUIRE(ok1, "renormalize_part: null pointer");
+      later = cD_layout;
+    } else {
+      ACETN_REQUIRE(later >= 1 && later <= cD_layout, "renormalize_part: count above its layout bound");
+      ACETN_REQUIRE(ok2, "renormalize_part: null pointer");
+    }
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
+      renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o, part, cD_layout,
+                  first);
     });
   });
 }

@@ -168,6 +168,29 @@ int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1
                                    const double* P2_j, int64_t cD_j, int precision, double* C_a,
                                    double* C_b, double* E_o, void* workspace, size_t ws_bytes,
                                    void* stream);
+/* The same renormalisation in two calls on one stream and one workspace, so
+ * that the work one projector pair feeds on its own starts before the other
+ * pair's projectors exist (the three functions of
+ * directional_mover.py:273-366, regrouped by pair).  ``first`` names the pair
+ * of part 1:
+ *   first = 0 (pair j): part 1 = C_b (renormalize_cj2) and E_o's
+ *     intermediate up to its double layer (P1_j contracted first, as
+ *     renormalize_ej does); part 2 = C_a (renormalize_cj1) and E_o (P2_i).
+ *   first = 1 (pair i): part 1 = C_a and E_o's intermediate with P2_i
+ *     contracted first; part 2 = C_b and E_o (P1_j last).
+ * Part 1 reads only its pair's projectors; the other count is then unknown
+ * and cD_layout (>= it, e.g. chi) fixes the workspace layout for both calls.
+ * Size the workspace with acetn_hip_renormalize_part_workspace, passing
+ * cD_layout in place of the count not yet known, and pass the same workspace
+ * to both parts.  Part 2 takes every projector and both counts. */
+size_t acetn_hip_renormalize_part_workspace(const acetn_site* s1, int k, int first, int64_t cD_i,
+                                            int64_t cD_j);
+int acetn_hip_renormalize_boundary_part(const acetn_site* s1, int k, int part, int first,
+                                        const double* P1_i, const double* P2_i, int64_t cD_i,
+                                        const double* P1_j, const double* P2_j, int64_t cD_j,
+                                        int64_t cD_layout, int precision, double* C_a,
+                                        double* C_b, double* E_o, void* workspace,
+                                        size_t ws_bytes, void* stream);
 
 /* One-site reduced density matrix rho[P][p] (d x d) of a site from its
  * C[0..3], E[0..3] and A.  Replaces RDM.build_site_rdm,

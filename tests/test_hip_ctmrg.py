@@ -226,3 +226,38 @@ def test_other_projector_branches_match_oracle(hip, projectors, svd_type, D, chi
     rel = 1e-9 if svd_type == "full-rank" else 1e-10
     print(f"{projectors}/{svd_type}: dE/E = {abs(e_h - e_o) / abs(e_o):.2e}")
     assert e_h == pytest.approx(e_o, rel=rel)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_renormalize_in_two_parts_matches_one_call(hip, precision):
+    """The two-part renormalisation, pair j's part first (P1_j contracted
+    first, as the reference does) or pair i's (P2_i first), the other count
+    only bounded at part 1, gives the one-call result for all four k."""
+    D, chi = 3, 12
+    A = oc.random_site_tensors(1, 1, D, 2, seed=41)[(0, 0)]
+    rng = np.random.default_rng(41)
+    C = [rng.standard_normal((chi, chi)) for _ in range(4)]
+    E = [rng.standard_normal((chi, chi, D, D)) for _ in range(4)]
+    site = (dev(A), [dev(c) for c in C], [dev(e) for e in E])
+    cdi, cdj = 9, 11
+    P1i, P2i = (dev(rng.standard_normal((chi * D * D, cdi))) for _ in range(2))
+    P1j, P2j = (dev(rng.standard_normal((chi * D * D, cdj))) for _ in range(2))
+    # FP64: only the split-K partition may differ; mixed: FP32 partial sums
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for k in range(4):
+        ref = hip.renormalize_boundary(site, k, P1i, P2i, P1j, P2j, precision=precision)
+        for first in ("j", "i"):
+            parts = hip.RenormalizeParts(site, k, chi, first_pair=first, slot=("test", k, first),
+                                         precision=precision)
+            if first == "j":
+                parts.first(P1j, P2j)
+                out = parts.second(P1i, P2i)
+            else:
+                parts.first(P1i, P2i)
+                out = parts.second(P1j, P2j)
+            torch.cuda.synchronize()
+            for name, a, b in zip(("C_a", "C_b", "E_o"), ref, out):
+                a, b = a.cpu().numpy(), b.cpu().numpy()
+                assert a.shape == b.shape, (k, first, name)
+                np.testing.assert_allclose(b, a, rtol=tol, atol=tol * np.abs(a).max(),
+                                           err_msg=f"{k} {first} {name}")

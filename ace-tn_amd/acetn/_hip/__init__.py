@@ -484,7 +484,8 @@ class RenormalizeParts:
     with the projectors of the pair named by ``first_pair`` ('j' or 'i'),
     then ``second(P1, P2)`` with the other pair's; ``second`` returns
     (C_a, C_b, E_o).  ``cd_bound`` >= the second pair's count (chi) fixes the
-    shared workspace layout."""
+    shared workspace layout.  With a ``slot`` or ``ws_key`` the workspace is
+    that key's, which no other call may use between the two parts."""
 
     def __init__(self, site1, k, cd_bound, first_pair="j", slot=None, precision="fp64",
                  ws_key=None):
@@ -510,7 +511,11 @@ class RenormalizeParts:
         cDi, cDj = (self.bound, cf) if self.first_code == 0 else (cf, self.bound)
         nb = lib().acetn_hip_renormalize_part_workspace(ctypes.byref(s1), k, self.first_code,
                                                         cDi, cDj)
-        self.ws = (workspace(nb, self.dev) if self.key is None
+        # the two calls share what part 1 leaves in the workspace: without a
+        # slot or key it is private to this object (the shared scratch could
+        # be reused by other calls in between)
+        self.ws = (torch.empty(max(int(nb), 256), dtype=torch.uint8, device=self.dev)
+                   if self.key is None
                    else slot_workspace(("renormalize", self.key), nb, self.dev))
         if self.first_code == 0:
             xc = s1.E_dim[k % 4][1]

@@ -1079,13 +1079,18 @@ __global__ void sumsq_kernel(const double* __restrict__ x, int64_t n, double* __
 
 __global__ void scale_by_norm_kernel(double* __restrict__ x, int64_t n,
                                      const double* __restrict__ part, int nparts) {
-  __shared__ double inv;
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < nparts; ++i) t += part[i];
-    inv = 1.0 / sqrt(t);
-  }
+  // every block sums the partials the same way: strided per thread, then a
+  // fixed tree (blockDim.x = 256)
+  __shared__ double red[256];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) t += part[i];
+  red[threadIdx.x] = t;
   __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double inv = 1.0 / sqrt(red[0]);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     x[i] *= inv;

@@ -1048,8 +1048,11 @@ __global__ void gauge_kernel(double* __restrict__ P1, double* __restrict__ P2, i
   const int z = blockIdx.x;
   __shared__ double red[256];
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < rows1; i += blockDim.x)
-    acc += (1.0 + fmod((double)i * 0.6180339887498949, 1.0)) * P1[i * cD + z];
+  // frac(x) = x - floor(x), exact for 0 <= x < 2^52 (the value fmod(x, 1) gives)
+  for (int64_t i = threadIdx.x; i < rows1; i += blockDim.x) {
+    const double x = (double)i * 0.6180339887498949;
+    acc += (1.0 + (x - floor(x))) * P1[i * cD + z];
+  }
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {

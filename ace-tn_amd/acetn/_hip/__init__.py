@@ -142,6 +142,9 @@ def _declare(lib):
         "acetn_hip_mlp_adam_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
                                                    ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                    ctypes.c_float, ctypes.c_float, c_vp]),
+        "acetn_hip_svd_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_svd": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp,
+                                         ctypes.POINTER(ctypes.c_int32), c_vp, c_size, c_vp]),
         "acetn_hip_eigh_workspace": (c_size, [c_i64]),
         "acetn_hip_eigh": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_orthonormalize_workspace": (c_size, [c_i64, c_i64]),
@@ -352,6 +355,26 @@ def eigh(G):
     ws = workspace(nb, G.device)
     check(lib().acetn_hip_eigh(ptr(G), q, ptr(w), ptr(V), ptr(ws), ws.numel(), stream_handle()))
     return w, V
+
+
+def svd(M):
+    """One-sided Jacobi SVD of a device FP64 matrix M (rows x cols):
+    (s, U, sweeps) with the min(rows, cols) leading singular values and the
+    left singular vectors as U's columns (rows x rows); sweeps < 0 means the
+    sweep limit stopped it."""
+    rows, cols = M.shape
+    Mt = M.t().contiguous()  # column j of M contiguous
+    s = torch.empty(rows, dtype=torch.float64, device=M.device)
+    U = torch.empty(rows, rows, dtype=torch.float64, device=M.device)
+    sweeps = ctypes.c_int32(0)
+    nb = lib().acetn_hip_svd_workspace(rows, cols)
+    if nb == 0:
+        raise ValueError(f"svd: unsupported shape {tuple(M.shape)}")
+    ws = workspace(nb, M.device)
+    check(lib().acetn_hip_svd(ptr(Mt), rows, cols, rows, ptr(s), ptr(U), ctypes.byref(sweeps),
+                              ptr(ws), ws.numel(), stream_handle()))
+    k = min(rows, cols)
+    return s[:k], U, sweeps.value
 
 
 def orthonormalize(Y):
@@ -601,8 +624,9 @@ def norm_tensor(site1, site2, k, a1q, a2q):
     dev = a1q.device
     n12 = _f64(nD, nD, nD, nD, device=dev)
     ws = workspace(lib().acetn_hip_norm_tensor_workspace(ctypes.byref(s1), ctypes.byref(s2), k), dev)
-    check(lib().acetn_hip_norm_tensor(ctypes.byref(s1), ctypes.byref(s2), k, ptr(a1q.contiguous()),
-                                      ptr(a2q.contiguous()), ptr(n12), ptr(ws), ws.numel(),
+    a1q, a2q = a1q.contiguous(), a2q.contiguous()  # held until the call has enqueued
+    check(lib().acetn_hip_norm_tensor(ctypes.byref(s1), ctypes.byref(s2), k, ptr(a1q),
+                                      ptr(a2q), ptr(n12), ptr(ws), ws.numel(),
                                       stream_handle()))
     return n12
 
@@ -611,8 +635,9 @@ def apply_gate(a1r, a2r, gate):
     nD, D, d = a1r.shape
     a12g = _f64(nD, nD, d, d, device=a1r.device)
     ws = workspace(lib().acetn_hip_apply_gate_workspace(D, d), a1r.device)
-    check(lib().acetn_hip_apply_gate(ptr(a1r.contiguous()), ptr(a2r.contiguous()),
-                                     ptr(gate.contiguous()), D, d, ptr(a12g), ptr(ws), ws.numel(),
+    a1r, a2r, gate = a1r.contiguous(), a2r.contiguous(), gate.contiguous()
+    check(lib().acetn_hip_apply_gate(ptr(a1r), ptr(a2r),
+                                     ptr(gate), D, d, ptr(a12g), ptr(ws), ws.numel(),
                                      stream_handle()))
     return a12g
 
@@ -623,7 +648,8 @@ def positive_approx(n12, D, d, cutoff):
     nz = _f64(nD, nD, nD * nD, device=n12.device)
     shifts = ctypes.c_int32(0)
     ws = workspace(lib().acetn_hip_positive_approx_workspace(D, d), n12.device)
-    check(lib().acetn_hip_positive_approx(ptr(n12.contiguous()), D, d, cutoff, ptr(nz),
+    n12 = n12.contiguous()
+    check(lib().acetn_hip_positive_approx(ptr(n12), D, d, cutoff, ptr(nz),
                                           ctypes.byref(shifts), ptr(ws), ws.numel(),
                                           stream_handle()))
     return nz, shifts.value
@@ -637,7 +663,8 @@ def gauge_fix(nz, a12g, D, d, atol):
     a12g_out = _f64(nD, nD, d, d, device=dev)
     nzxr_inv, nzyr_inv = _f64(nD, nD, device=dev), _f64(nD, nD, device=dev)
     ws = workspace(lib().acetn_hip_gauge_fix_workspace(D, d), dev)
-    check(lib().acetn_hip_gauge_fix(ptr(nz.contiguous()), ptr(a12g.contiguous()), D, d, atol,
+    nz, a12g = nz.contiguous(), a12g.contiguous()
+    check(lib().acetn_hip_gauge_fix(ptr(nz), ptr(a12g), D, d, atol,
                                     ptr(n12), ptr(a12g_out), ptr(nzxr_inv), ptr(nzyr_inv), ptr(ws),
                                     ws.numel(), stream_handle()))
     return n12, a12g_out, nzxr_inv, nzyr_inv
@@ -649,7 +676,8 @@ def als_initialize(a12g, D):
     dev = a12g.device
     a1r, a2r = _f64(nD, D, d, device=dev), _f64(nD, D, d, device=dev)
     ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)
-    check(lib().acetn_hip_als_initialize(ptr(a12g.contiguous()), D, d, ptr(a1r), ptr(a2r), ptr(ws),
+    a12g = a12g.contiguous()
+    check(lib().acetn_hip_als_initialize(ptr(a12g), D, d, ptr(a1r), ptr(a2r), ptr(ws),
                                          ws.numel(), stream_handle()))
     return a1r, a2r
 
@@ -666,7 +694,8 @@ def als_solve(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
     b1, b2 = a1r.clone().contiguous(), a2r.clone().contiguous()
     sweeps, cost, info = c_i64(0), c_dbl(0.0), ctypes.c_int32(0)
     ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)
-    check(lib().acetn_hip_als_solve(ptr(n12.contiguous()), ptr(a12g.contiguous()), ptr(b1),
+    n12, a12g = n12.contiguous(), a12g.contiguous()
+    check(lib().acetn_hip_als_solve(ptr(n12), ptr(a12g), ptr(b1),
                                     ptr(b2), D, d, int(niter), float(tol), ALS_METHODS[method],
                                     float(epsilon), ctypes.byref(sweeps), ctypes.byref(cost),
                                     ctypes.byref(info), ptr(ws), ws.numel(), stream_handle()))
@@ -680,10 +709,13 @@ def finalize_bond(a1r, a2r, nzyr_inv, nzxr_inv, a1q, a2q, k):
     dev = a1r.device
     A1, A2 = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
     ws = workspace(lib().acetn_hip_finalize_bond_workspace(D, d), dev)
-    gy = ptr(nzyr_inv.contiguous()) if nzyr_inv is not None else None
-    gx = ptr(nzxr_inv.contiguous()) if nzxr_inv is not None else None
-    check(lib().acetn_hip_finalize_bond(ptr(a1r.contiguous()), ptr(a2r.contiguous()), gy, gx,
-                                        ptr(a1q.contiguous()), ptr(a2q.contiguous()), k, D, d,
+    nzyr_inv = nzyr_inv.contiguous() if nzyr_inv is not None else None
+    nzxr_inv = nzxr_inv.contiguous() if nzxr_inv is not None else None
+    a1r, a2r, a1q, a2q = (t.contiguous() for t in (a1r, a2r, a1q, a2q))
+    gy = ptr(nzyr_inv) if nzyr_inv is not None else None
+    gx = ptr(nzxr_inv) if nzxr_inv is not None else None
+    check(lib().acetn_hip_finalize_bond(ptr(a1r), ptr(a2r), gy, gx,
+                                        ptr(a1q), ptr(a2q), k, D, d,
                                         ptr(A1), ptr(A2), ptr(ws), ws.numel(), stream_handle()))
     return A1, A2
 
@@ -695,15 +727,16 @@ def simple_update_bond(A1, A2, k, lam1, lam2, gate, cutoff=1e-12):
     dev = A1.device
     A1o, A2o = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
     lam = _f64(D, device=dev)
-    keep = [t.contiguous() for t in list(lam1) + list(lam2) if t is not None]
-    l1 = (c_vp * 4)(*[ptr(t.contiguous()) if t is not None else None for t in lam1])
-    l2 = (c_vp * 4)(*[ptr(t.contiguous()) if t is not None else None for t in lam2])
+    lam1 = [t.contiguous() if t is not None else None for t in lam1]
+    lam2 = [t.contiguous() if t is not None else None for t in lam2]
+    A1, A2, gate = A1.contiguous(), A2.contiguous(), gate.contiguous()
+    l1 = (c_vp * 4)(*[ptr(t) if t is not None else None for t in lam1])
+    l2 = (c_vp * 4)(*[ptr(t) if t is not None else None for t in lam2])
     ws = workspace(lib().acetn_hip_simple_update_workspace(D, d), dev)
-    check(lib().acetn_hip_simple_update_bond(ptr(A1.contiguous()), ptr(A2.contiguous()), k, l1, l2,
-                                             ptr(gate.contiguous()), D, d, cutoff, ptr(A1o),
+    check(lib().acetn_hip_simple_update_bond(ptr(A1), ptr(A2), k, l1, l2,
+                                             ptr(gate), D, d, cutoff, ptr(A1o),
                                              ptr(A2o), ptr(lam), ptr(ws), ws.numel(),
                                              stream_handle()))
-    del keep
     return A1o, A2o, lam
 
 

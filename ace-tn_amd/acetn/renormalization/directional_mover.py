@@ -124,9 +124,10 @@ def run_stages(items):
     """One stage of each of several independent networks, run as one round.
 
     ``items`` is a list of (mover, ipeps, moves).  Every projector pair of
-    every item is started first (a single network's pair i on slot i, the
-    pairs of a multi-network round on slot g % N_SLOTS in round order g, each
-    with a workspace of its own); the host then reads each pair's truncation
+    every item is started first (pair g of the round, counted over all
+    items, on slot g % N_SLOTS with a workspace of its own, so that every
+    stream used is one of the slot streams that waited on the caller's
+    stream); the host then reads each pair's truncation
     count as soon as that pair has produced it, enqueues its projector
     products, and starts each renormalisation in two parts: the part that
     the first finished of its two pairs feeds on its own at once, the rest
@@ -142,7 +143,6 @@ def run_stages(items):
         for m, net, mv in items:
             m.stage(net, mv)
         return
-    multi = len(items) > 1
     plans, base, joined = [], 0, set()
     for mover, ipeps, moves in items:
         jobs, targets = mover._plan(ipeps, moves)
@@ -157,7 +157,7 @@ def run_stages(items):
                 ev.record(torch.cuda.current_stream(dev))
                 for sl in range(1, _hip.N_SLOTS):
                     _hip.slot_stream(sl, dev).wait_event(ev)
-        started = calc.start_many(ipeps, jobs, base=base if multi else None)
+        started = calc.start_many(ipeps, jobs, base=base)
         plans.append({"mover": mover, "ipeps": ipeps, "targets": targets, "started": started,
                       "base": base, "proj": [None] * len(jobs), "done": [None] * len(jobs),
                       "results": [None] * len(targets)})
@@ -174,7 +174,7 @@ def run_stages(items):
                 if results[t] is not None:
                     continue
                 g = pl["base"] + t
-                slot = g % _hip.N_SLOTS if multi else t
+                slot = g % _hip.N_SLOTS
                 stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
                 # the part that the first finished of the two pairs feeds on
                 # its own starts at once, the rest when the other pair is done
@@ -187,7 +187,7 @@ def run_stages(items):
                         parts[t] = (_hip.RenormalizeParts(
                             (st['A'], st['C'], st['E']), k, ipeps.dims['chi'],
                             first_pair="j" if a == j else "i", slot=slot,
-                            precision=calc.precision, ws_key=g if multi else None), a)
+                            precision=calc.precision, ws_key=g), a)
                         parts[t][0].first(flat(proj[a][0]), flat(proj[a][1]))
                 if parts[t] is not None and proj[i] is not None and proj[j] is not None:
                     b = i if parts[t][1] == j else j

@@ -65,12 +65,12 @@ class ProjectorCalculator:
         torch.cuda.synchronize()
         return [pr for _, pr, _ in out]
 
-    def start_many(self, ipeps, jobs, system=None, base=None):
+    def start_many(self, ipeps, jobs, system=None, base=0):
         """Enqueue every projector pair up to its truncation count; returns
-        the started jobs in ``jobs`` order.  Pair i runs on slot i (one
-        stream per pair); with ``base`` set (several networks in one round),
-        pair i is round-wide pair base + i on slot (base + i) % N_SLOTS with
-        a workspace of its own."""
+        the started jobs in ``jobs`` order.  Pair i is round-wide pair
+        base + i (``base`` counts the pairs of networks started before it in
+        the same round) and runs on slot (base + i) % N_SLOTS with a
+        workspace of its own."""
         system = system or ("full-system" if self.calculate == self.calculate_full_system
                             else "half-system")
         if self.svd_type not in ("rsvd", "full-rank"):
@@ -78,8 +78,8 @@ class ProjectorCalculator:
         chi = ipeps.dims['chi']
         started = []
         for i, (sites, k) in enumerate(jobs):
-            slot = i if base is None else (base + i) % _hip.N_SLOTS
-            ws_key = None if base is None else base + i
+            slot = (base + i) % _hip.N_SLOTS
+            ws_key = base + i
             sites = [tuple(s) for s in sites]
             for s in sites:
                 ipeps[s]  # raises ValueError for an unknown site

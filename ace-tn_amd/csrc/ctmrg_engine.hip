@@ -584,8 +584,14 @@ void renormalize(Ctx& c, const acetn_site* s1, int k, const double* P1i, const d
     h.cn = r1 ? plain(D2) : plain(cDj * D2);
     h.alpha = 1.0; h.beta = 0.0;
     // mixed precision: T only feeds the FP32 double layer, so it is stored in
-    // FP32, and so is Z when the last product splits it into planes
-    const bool t32 = c.mixed, z32 = c.mixed && gemm_split_is_planar(h);
+    // FP32, and so is Z when the last product splits it into planes.  Z is
+    // written by part 1 and read by part 2, which sees the real count where
+    // part 1 saw its layout bound: the path (and with it Z's element type)
+    // is chosen from the bound in both parts.
+    GemmDesc hp = h;
+    if (part != 0) hp.N = (int)cD_layout;
+    const bool t32 = c.mixed, z32 = c.mixed && gemm_split_is_planar(hp);
+    h.planar = z32 ? 1 : 0;
     h.a_f32 = z32;
     if (t32 && c.launch) {
       GemmBatch b{};
@@ -1156,6 +1162,28 @@ int acetn_hip_gemm_split(int64_t M, int64_t N, int64_t K, int ta, const double* 
       c.split = pieces;
       mm(c, (int)M, (int)N, (int)K, ta != 0, A, lda, tb != 0, B, ldb, C, ldc, alpha, beta);
     });
+  });
+}
+
+size_t acetn_hip_svd_workspace(int64_t rows, int64_t cols) {
+  if (rows < 1 || cols < 1 || rows > 2048 || cols > 2048) return 0;
+  return sizeof(double) * jacobi_svd_scratch_elems((int)rows, (int)cols) + 512;
+}
+
+int acetn_hip_svd(const double* M, int64_t rows, int64_t cols, int64_t ldc, double* s, double* U,
+                  int32_t* host_sweeps, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    ACETN_REQUIRE(M && s && U && ws, "svd: null pointer");
+    ACETN_REQUIRE(rows >= 1 && cols >= 1 && rows <= 2048 && cols <= 2048 && ldc >= rows,
+                  "svd: bad shape");
+    ACETN_REQUIRE(ws_bytes + 512 >= acetn_hip_svd_workspace(rows, cols), "svd: workspace too small");
+    double* scr = reinterpret_cast<double*>(((uintptr_t)ws + 255) & ~uintptr_t(255));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    jacobi_svd(M, (int)rows, (int)cols, ldc, s, U, scr, st);
+    int info[2] = {0, 0};
+    ACETN_HIP_CHECK(hipMemcpyAsync(info, scr, sizeof(info), hipMemcpyDeviceToHost, st));
+    ACETN_HIP_CHECK(hipStreamSynchronize(st));
+    if (host_sweeps) *host_sweeps = info[1] ? info[0] : -info[0];
   });
 }
 

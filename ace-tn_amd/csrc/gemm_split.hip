@@ -844,7 +844,10 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
 
 int split_planes_kp(int K) { return padded_k(K); }
 
-bool gemm_split_is_planar(const GemmDesc& g) { return 2.0 * g.M * g.N * (double)g.K >= kPlanarFlops; }
+bool gemm_split_is_planar(const GemmDesc& g) {
+  if (g.planar >= 0) return g.planar != 0;
+  return 2.0 * g.M * g.N * (double)g.K >= kPlanarFlops;
+}
 
 size_t split_planes_elems(int rows, int K) {
   const size_t bytes = (size_t)3 * rows * padded_k(K) * sizeof(__bf16);

@@ -83,6 +83,8 @@ struct GemmDesc {
   const int* run_if = nullptr;  // device flag: when given and zero the product is skipped
   bool c_f32 = false;           // split-bf16 products only: C holds FP32 (the pointer is a float*)
   bool a_f32 = false;           // split-bf16 products on the plane path only: A holds FP32
+  int8_t planar = -1;           // split-bf16 products: 1 / 0 forces the plane / general path,
+                                // -1 picks by size (gemm_split_is_planar)
 };
 
 struct GemmBatch {
@@ -206,6 +208,17 @@ void refill_dead_columns(double* Y, int64_t m, int q, const int* dead, uint64_t 
 void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s,
                  double tol = 2.2e-16);
 size_t jacobi_rot_elems(int q);
+
+// One-sided (Hestenes) Jacobi SVD of the rows x cols matrix M whose column j
+// is stored contiguously at Mc + j * ldc (M(i, j) = Mc[j * ldc + i]),
+// max(rows, cols) <= 2048.  Writes the rows leading singular values,
+// descending, to sig (zeros past the rank) and the matching left singular
+// vectors as the columns of U (row-major rows x rows, U[i * rows + k]).
+// ``scratch`` holds jacobi_svd_scratch_elems(rows, cols) doubles; its first
+// two ints receive the sweep count and a converged flag.
+void jacobi_svd(const double* Mc, int rows, int cols, int64_t ldc, double* sig, double* U,
+                double* scratch, hipStream_t st);
+size_t jacobi_svd_scratch_elems(int rows, int cols);
 
 // Column scaling / projector finishing kernels.
 // X[i, z] = Ub[i, z] * f(s[z]) for z < cD, with f selected by ``mode``:

@@ -334,9 +334,20 @@ int acetn_hip_gemm_split(int64_t M, int64_t N, int64_t K, int ta, const double* 
                          int tb, const double* B, int64_t ldb, double alpha, double beta, double* C,
                          int64_t ldc, int pieces, void* workspace, size_t ws_bytes, void* stream);
 
+/* One-sided (Hestenes) Jacobi SVD of the rows x cols matrix M whose column j
+ * is contiguous at M + j * ldc; max(rows, cols) <= 2048.  Writes the ``rows``
+ * leading singular values, descending (zero past the rank), to s and the
+ * left singular vectors as the columns of U (rows x rows, row-major);
+ * *host_sweeps receives the number of sweeps (negative when the sweep limit
+ * stopped it).  Synchronises the stream.  The projector SVD runs the same
+ * kernels (torch.linalg.svd at acetn/linalg/fused_matmul_svd_lowrank.py:48
+ * and acetn/renormalization/projectors.py:219-236). */
+size_t acetn_hip_svd_workspace(int64_t rows, int64_t cols);
+int acetn_hip_svd(const double* M, int64_t rows, int64_t cols, int64_t ldc, double* s, double* U,
+                  int32_t* host_sweeps, void* workspace, size_t ws_bytes, void* stream);
+
 /* Symmetric eigen-decomposition (descending) of a q x q matrix (G is
- * overwritten).  Replaces torch.linalg.eigh at acetn/evolution/full_update.py:283
- * and backs the projector SVD. */
+ * overwritten).  Replaces torch.linalg.eigh at acetn/evolution/full_update.py:283. */
 size_t acetn_hip_eigh_workspace(int64_t q);
 int acetn_hip_eigh(double* G, int64_t q, double* w, double* V, void* workspace,
                    size_t ws_bytes, void* stream);

@@ -15,7 +15,7 @@ Instead:
   * storages appear in serialization order: per site (x-major site list)
     A, C0..C3, E0..E3.
 The resulting tensors are validated by the golden energies (rel 1e-10) in
-tests/test_oracle_golden.py.
+tests/test_oracle.py::test_measure_matches_golden_energy.
 
 Usage:  python tests/golden/make_acetn_gs_fixtures.py /root/reference
 """

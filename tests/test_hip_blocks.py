@@ -112,6 +112,71 @@ def test_eigh_graded_and_degenerate(hip):
     np.testing.assert_allclose(V.T @ V, np.eye(40), atol=1e-12)
 
 
+def _svd_checks(hip, M, rtol_vec=1e-12):
+    """Singular values against LAPACK (absolute eps-class accuracy, as a
+    backward-stable SVD has), orthonormal left vectors, and U^T M = S V^T
+    (the rows of U^T M are orthogonal with norms s)."""
+    rows, cols = M.shape
+    s, U, sweeps = hip.svd(dev(M))
+    s, U = s.cpu().numpy(), U.cpu().numpy()
+    sr = np.linalg.svd(M, compute_uv=False)
+    k = min(rows, cols)
+    assert sweeps > 0, f"no convergence ({-sweeps} sweeps)"
+    np.testing.assert_allclose(s, sr[:k], rtol=0, atol=4 * max(rows, cols) * 2.2e-16 * sr[0])
+    # a singular vector is resolved to about eps s0 / s_k (as by LAPACK):
+    # vector checks on the values above 1e-4 s0, scaled by s0 / s_min there
+    live = s > 1e-4 * s[0]
+    Ul = U[:, :k][:, live]
+    amp = s[0] / s[live].min()
+    np.testing.assert_allclose(Ul.T @ Ul, np.eye(Ul.shape[1]), atol=1e-14 * amp + 1e-13)
+    W = Ul.T @ M   # = S V^T
+    np.testing.assert_allclose(np.linalg.norm(W, axis=1), s[live], rtol=0,
+                               atol=(1e-14 * amp + 1e-13) * sr[0])
+    G = W @ W.T
+    np.testing.assert_allclose(G - np.diag(np.diag(G)), 0, atol=(1e-14 * amp + 1e-13) * sr[0] ** 2)
+    return s, U, sweeps
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (5, 5), (22, 22), (66, 66), (146, 146), (150, 146),
+                                       (146, 160), (192, 192), (202, 202), (272, 272), (300, 280)])
+def test_jacobi_svd_matches_lapack(hip, rows, cols):
+    rng = np.random.default_rng(rows * 1000 + cols)
+    M = rng.standard_normal((rows, cols))
+    s, U, sweeps = _svd_checks(hip, M)
+    print(f"{rows}x{cols}: {sweeps} sweeps")
+    # well-separated random spectrum: the vectors themselves match LAPACK's up to sign
+    Ur = np.linalg.svd(M)[0]
+    k = min(rows, cols)
+    d = np.abs(np.sum(U[:, :k] * Ur[:, :k], axis=0))
+    sr = np.linalg.svd(M, compute_uv=False)
+    gap = np.minimum(np.abs(np.diff(np.r_[np.inf, sr])), np.abs(np.diff(np.r_[sr, -np.inf])))[:k]
+    ok = gap > 1e-3 * sr[0]
+    assert np.all(1 - d[ok] < 1e-14 + 1e-12 * (sr[0] / gap[ok]) ** 2)
+
+
+def test_jacobi_svd_graded_degenerate_and_rank_deficient(hip):
+    rng = np.random.default_rng(11)
+    # graded spectrum down to 1e-14 relative with an exact 3-fold degeneracy
+    q = 146
+    U0, _ = np.linalg.qr(rng.standard_normal((q, q)))
+    V0, _ = np.linalg.qr(rng.standard_normal((q, q)))
+    sv = np.concatenate([[1, 1, 1], np.logspace(-1, -14, q - 3)])
+    s, U, _ = _svd_checks(hip, (U0 * sv) @ V0.T)
+    # values below q eps |M| are roundoff (the Jacobi sweeps leave columns
+    # that small alone): absolute accuracy q eps |M|_F
+    np.testing.assert_allclose(s, sv, rtol=0, atol=q * 1.1e-16 * np.sqrt(3.0))
+    # rank 40 of 120 (zero singular values past the rank)
+    M = rng.standard_normal((120, 40)) @ rng.standard_normal((40, 120))
+    s, U, _ = _svd_checks(hip, M)
+    assert np.all(s[40:] <= 1e-12 * s[0])
+    # a zero matrix and a single nonzero column
+    s, U, _ = hip.svd(dev(np.zeros((7, 7))))
+    assert float(s.abs().max()) == 0.0
+    M = np.zeros((9, 9)); M[:, 4] = rng.standard_normal(9)
+    s, U, _ = _svd_checks(hip, M)
+    np.testing.assert_allclose(s[0], np.linalg.norm(M[:, 4]), rtol=1e-15)
+
+
 @pytest.mark.parametrize("m,q,rank", [(200, 20, 20), (1024, 66, 66), (300, 30, 1), (80, 22, 16),
                                       (16, 16, 12), (600, 100, 100), (2000, 146, 146),
                                       (1500, 146, 60), (2000, 170, 170), (900, 173, 150),

@@ -229,17 +229,20 @@ def test_other_projector_branches_match_oracle(hip, projectors, svd_type, D, chi
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_renormalize_in_two_parts_matches_one_call(hip, precision):
+@pytest.mark.parametrize("D,chi,cdi,cdj", [(3, 12, 9, 11), (4, 64, 40, 64), (4, 64, 64, 40)])
+def test_renormalize_in_two_parts_matches_one_call(hip, precision, D, chi, cdi, cdj):
     """The two-part renormalisation, pair j's part first (P1_j contracted
     first, as the reference does) or pair i's (P2_i first), the other count
-    only bounded at part 1, gives the one-call result for all four k."""
-    D, chi = 3, 12
+    only bounded at part 1, gives the one-call result for all four k.  At
+    D=4, chi=64 the last edge product at the layout bound (64 columns) is
+    large enough for the split-bf16 plane path and at 40 columns it is not,
+    so part 2 must keep the path (and the element type of the intermediate)
+    that part 1 chose."""
     A = oc.random_site_tensors(1, 1, D, 2, seed=41)[(0, 0)]
     rng = np.random.default_rng(41)
     C = [rng.standard_normal((chi, chi)) for _ in range(4)]
     E = [rng.standard_normal((chi, chi, D, D)) for _ in range(4)]
     site = (dev(A), [dev(c) for c in C], [dev(e) for e in E])
-    cdi, cdj = 9, 11
     P1i, P2i = (dev(rng.standard_normal((chi * D * D, cdi))) for _ in range(2))
     P1j, P2j = (dev(rng.standard_normal((chi * D * D, cdj))) for _ in range(2))
     # FP64: only the split-K partition may differ; mixed: FP32 partial sums
@@ -247,8 +250,8 @@ def test_renormalize_in_two_parts_matches_one_call(hip, precision):
     for k in range(4):
         ref = hip.renormalize_boundary(site, k, P1i, P2i, P1j, P2j, precision=precision)
         for first in ("j", "i"):
-            parts = hip.RenormalizeParts(site, k, chi, first_pair=first, slot=("test", k, first),
-                                         precision=precision)
+            parts = hip.RenormalizeParts(site, k, chi, first_pair=first,
+                                         slot=("test", k, first, chi), precision=precision)
             if first == "j":
                 parts.first(P1j, P2j)
                 out = parts.second(P1i, P2i)
@@ -261,3 +264,23 @@ def test_renormalize_in_two_parts_matches_one_call(hip, precision):
                 assert a.shape == b.shape, (k, first, name)
                 np.testing.assert_allclose(b, a, rtol=tol, atol=tol * np.abs(a).max(),
                                            err_msg=f"{k} {first} {name}")
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 2), (3, 3)])
+def test_ctmrg_larger_unit_cell_matches_oracle(hip, nx, ny):
+    """Unit cells with more than two sites along a line: a stage then has
+    more projector pairs than stream slots, so pairs share slots (each with
+    its own workspace) and every slot stream must be ordered after the
+    caller's stream, where the boundary tensors were uploaded."""
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+    D, chi, seed = 2, 8, 606
+    st0 = oc.make_state(oc.random_site_tensors(nx, ny, D, 2, seed=60 + nx * ny, bias=0.3))
+    ost = copy.deepcopy(st0)
+    omover = oc.DirectionalMover(oc.ProjectorCalculator(omega_fn=SketchStream(seed)), nx, ny, chi)
+    hst = DevState(st0, nx, ny, D, chi, seed)
+    hmover = DirectionalMover(Cfg(), hst.sketch)
+    for _ in range(2):
+        oc.ctmrg_iteration(omover, ost)
+        ctmrg_iteration(hmover, hst)
+        assert_state_close(hst.to_numpy(), ost, rtol=1e-9)

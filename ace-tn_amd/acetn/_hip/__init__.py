@@ -125,6 +125,10 @@ def _declare(lib):
                                                ctypes.POINTER(c_dbl),
                                                ctypes.POINTER(ctypes.c_int32), c_vp, c_size,
                                                c_vp]),
+        "acetn_hip_als_solve_noalloc": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
+                                                       ctypes.c_int, c_dbl, ctypes.c_int, c_dbl,
+                                                       ctypes.POINTER(c_i64), ctypes.POINTER(c_dbl),
+                                                       ctypes.POINTER(ctypes.c_int32), c_vp]),
         "acetn_hip_finalize_bond_workspace": (c_size, [c_i64, c_i64]),
         "acetn_hip_finalize_bond": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                                    c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
@@ -700,6 +704,25 @@ def als_solve(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
                                     float(epsilon), ctypes.byref(sweeps), ctypes.byref(cost),
                                     ctypes.byref(info), ptr(ws), ws.numel(), stream_handle()))
     return b1, b2, sweeps.value, cost.value, info.value
+
+
+def als_solve_noalloc(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
+    """als_solve through the entry point that allocates nothing (its scratch
+    is a fixed device array of the library, nD * D <= 128); a1r/a2r are
+    updated in place.  Returns (sweeps, cost, flags)."""
+    nD, D, d = a1r.shape
+    if method not in ALS_METHODS:
+        raise ValueError(f"Unknown ALS method: {method}")
+    for t in (n12, a12g, a1r, a2r):
+        if not t.is_contiguous():
+            raise ValueError("als_solve_noalloc: tensors must be contiguous")
+    sweeps, cost, info = c_i64(0), c_dbl(0.0), ctypes.c_int32(0)
+    check(lib().acetn_hip_als_solve_noalloc(ptr(n12), ptr(a12g), ptr(a1r), ptr(a2r), D, d,
+                                            int(niter), float(tol), ALS_METHODS[method],
+                                            float(epsilon), ctypes.byref(sweeps),
+                                            ctypes.byref(cost), ctypes.byref(info),
+                                            stream_handle()))
+    return sweeps.value, cost.value, info.value
 
 
 def finalize_bond(a1r, a2r, nzyr_inv, nzxr_inv, a1q, a2q, k):

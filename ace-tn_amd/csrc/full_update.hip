@@ -6,6 +6,9 @@
 #include "engine.h"
 #include "fu_internal.h"
 
+#include <mutex>
+#include <vector>
+
 namespace acetn {
 namespace {
 
@@ -357,6 +360,39 @@ void simple_update_bond(Ctx& c, const double* A1, const double* A2, int k,
 
 using namespace acetn;
 
+namespace {
+
+// Scratch of acetn_hip_als_solve_noalloc: the ALS system matrix and
+// right-hand sides of the largest problem it accepts (nD * D <= 128 rows,
+// d <= 4), then the final cost and the sweep / flag words.  One copy per
+// device, held until the process ends; the kernel writes every element it
+// reads, so nothing has to start at zero.
+constexpr int kNoAllocMaxNR = 128;
+constexpr size_t kNoAllocWork = (size_t)kNoAllocMaxNR * kNoAllocMaxNR + (size_t)kNoAllocMaxNR * 4;
+constexpr size_t kNoAllocElems = kNoAllocWork + 4 + 4;
+__device__ __align__(256) double scratch[kNoAllocElems];
+
+// the device address of ``scratch`` on each device, looked up once per
+// device with that device current (the host-side name is only a stand-in)
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;
+
+double* noalloc_scratch() {
+  int dev = 0;
+  ACETN_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    ACETN_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ACETN_REQUIRE(p != nullptr, "als_solve_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = reinterpret_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
 extern "C" {
 
 size_t acetn_hip_bond_decompose_workspace(int64_t D, int64_t d) {
@@ -509,6 +545,39 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
       if (host_cost) *host_cost = hc;
       if (host_info) *host_info = hi[1];
     });
+  });
+}
+
+int acetn_hip_als_solve_noalloc(const double* n12, const double* a12g, double* a1r, double* a2r,
+                                int64_t D, int64_t d, int niter, double tol, int method,
+                                double epsilon, int64_t* host_sweeps, double* host_cost,
+                                int32_t* host_info, void* stream) {
+  return guarded([&] {
+    const FuDims f = fu_dims(D, d);
+    ACETN_REQUIRE(n12 && a12g && a1r && a2r, "als_solve_noalloc: null pointer");
+    ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY,
+                  "als_solve_noalloc: only the 'cholesky' method (with its LU fallback) runs on "
+                  "the device");
+    ACETN_REQUIRE(niter >= 0, "als_solve_noalloc: negative niter");
+    ACETN_REQUIRE(f.nD * f.D <= kNoAllocMaxNR && f.d <= 4,
+                  "als_solve_noalloc: nD * D must be at most 128 (the fixed scratch holds the "
+                  "system of D = 8, d = 2); use acetn_hip_als_solve for larger bonds");
+    ACETN_REQUIRE(als_work_elems((int)f.nD, (int)f.D, (int)f.d) <= kNoAllocWork,
+                  "als_solve_noalloc: problem exceeds the fixed scratch");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    double* base = noalloc_scratch();
+    double* cost = base + kNoAllocWork;
+    int* info = reinterpret_cast<int*>(base + kNoAllocWork + 4);
+    als_solve_device(n12, a12g, a1r, a2r, (int)f.nD, (int)f.D, (int)f.d, niter, tol, epsilon, base,
+                     cost, info, st);
+    double hc = 0.0;
+    int hi[2] = {0, 0};
+    ACETN_HIP_CHECK(hipMemcpyAsync(&hc, cost, sizeof(double), hipMemcpyDeviceToHost, st));
+    ACETN_HIP_CHECK(hipMemcpyAsync(hi, info, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    ACETN_HIP_CHECK(hipStreamSynchronize(st));
+    if (host_sweeps) *host_sweeps = hi[0];
+    if (host_cost) *host_cost = hc;
+    if (host_info) *host_info = hi[1];
   });
 }
 

@@ -266,6 +266,18 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
                         int64_t* host_sweeps, double* host_cost, int32_t* host_info,
                         void* workspace, size_t ws_bytes, void* stream);
 
+/* acetn_hip_als_solve without a workspace argument, for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * scratch is a fixed device array of the library (about 133 KB per device,
+ * held until the process ends) sized for nD * D <= 128 (D = 8 at d = 2);
+ * larger bonds return an error.  Calls that overlap on one device share that
+ * array and race: serialise them. */
+int acetn_hip_als_solve_noalloc(const double* n12, const double* a12g, double* a1r, double* a2r,
+                                int64_t D, int64_t d, int niter, double tol, int method,
+                                double epsilon, int64_t* host_sweeps, double* host_cost,
+                                int32_t* host_info, void* stream);
+
 /* finalize_reduced_tensors (full_update.py:123-161; nzyr_inv/nzxr_inv null
  * when the gauge fix is off) + recompose_site_tensors (tensor_update.py:70-75),
  * each tensor divided by its norm and rotated back to the site frame

@@ -292,7 +292,7 @@ def kernel_timing(enable):
 
 
 # kernel classes of acetn_hip_kernel_timing_read
-K_GEMM_F64, K_GEMM_SPLIT, K_DOUBLE_LAYER = 0, 1, 2
+K_GEMM_F64, K_GEMM_SPLIT, K_DOUBLE_LAYER, K_JACOBI, K_CHOLESKY = 0, 1, 2, 3, 4
 # precision modes (ACETN_HIP_FP64 / ACETN_HIP_MIXED)
 PRECISIONS = {"fp64": 0, "mixed": 1}
 

@@ -2,6 +2,8 @@
 HIP engine.  The enlarged corners, the randomized range finder, the FP64
 Jacobi SVD and the projector products all run on the GPU
 (acetn_hip_half_projectors)."""
+import collections
+
 from .. import _hip
 
 
@@ -31,6 +33,9 @@ class ProjectorCalculator:
         self.precision = getattr(config, "precision", "fp64")
         _hip.precision_code(self.precision)  # validates
         self.last_spectrum = None
+        # (spectrum, kept count) of the most recent projector pairs, newest
+        # last: the singular values the truncation chooses from
+        self.recent_spectra = collections.deque(maxlen=256)
         self.set_calculate()
 
     def set_calculate(self):
@@ -101,10 +106,26 @@ class ProjectorCalculator:
         for idx, job in enumerate(started):
             job.ready.synchronize()
             P1, P2 = job.finish()
+            self.recent_spectra.append((job.spec, P1.shape[1]))
             done = torch.cuda.Event()
             done.record(job.stream)
             rows = P1.shape[0]
             yield idx, (P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)), done
         if started:
             self.last_spectrum = started[-1].spec
+
+    def truncation_errors(self, count):
+        """Projector truncation error of the ``count`` most recent pairs: the
+        share of the computed spectrum's weight that the truncation drops,
+        sum_{i >= cD} s_i^2 / sum_i s_i^2 over the q = chi + oversampling
+        singular values of the half-system matrix (the reference keeps
+        cD = min(chi, #{s/s0 > cutoff}) of them, projectors.py:163-164, and
+        computes no error)."""
+        import torch
+        out = []
+        for spec, cd in list(self.recent_spectra)[-count:]:
+            s2 = spec.double().cpu() ** 2
+            tot = float(s2.sum())
+            out.append(float(s2[cd:].sum()) / tot if tot > 0 else 0.0)
+        return out
 

@@ -136,7 +136,14 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
 
 // Live timing of the matrix-core kernels (HIP events on the launch stream),
 // per kernel class; the benchmark's roofline figures come from here.
-enum KernelClass : int { kKGemmF64 = 0, kKGemmSplit = 1, kKDoubleLayer = 2, kKClasses = 3 };
+enum KernelClass : int {
+  kKGemmF64 = 0,
+  kKGemmSplit = 1,
+  kKDoubleLayer = 2,
+  kKJacobi = 3,    // projector eigensolver (one workgroup): rotations + their replay on V
+  kKCholesky = 4,  // CholeskyQR factor and inverse (one workgroup)
+  kKClasses = 5
+};
 void kernel_timing_enable(bool on);
 void kernel_timing_read(int cls, double* ms, double* flops, int64_t* launches);
 // Returns an event recorded on ``s`` (nullptr when timing is off); pass it to

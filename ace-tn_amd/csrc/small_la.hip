@@ -1122,6 +1122,12 @@ void gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed, uint64_t 
 void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double rank_tol,
                hipStream_t s, double* gscratch, int* dead_out, const int* only_if_dead) {
   ACETN_REQUIRE(q >= 1 && q <= 512, "chol_rinv: q out of range");
+  // q^3/3 (factor) + q^3/6 (inverse)
+  hipEvent_t e0 = kernel_timing_begin(s);
+  struct End {
+    hipEvent_t e0; hipStream_t s; double f;
+    ~End() { if (e0) kernel_timing_end(e0, s, kKCholesky, f, 0.0); }
+  } timing_end{e0, s, 0.5 * (double)q * q * q};
   const size_t pb = packed_bytes(q);
   if (q <= kCbMaxQ) {
     const int Q = (q + 15) / 16;
@@ -1176,6 +1182,11 @@ size_t jacobi_rot_elems(int q) {
 
 void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_t s, double tol) {
   ACETN_REQUIRE(q >= 1 && q <= 510, "jacobi_eigh: q out of range");
+  hipEvent_t e0 = kernel_timing_begin(s);
+  struct End {
+    hipEvent_t e0; hipStream_t s;
+    ~End() { if (e0) kernel_timing_end(e0, s, kKJacobi, 0.0, 0.0); }
+  } timing_end{e0, s};
   const int n = q + (q & 1);
   double* wraw = rot + jacobi_record_elems(q);
   int* info = reinterpret_cast<int*>(wraw + q);

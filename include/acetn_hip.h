@@ -71,9 +71,18 @@ int acetn_hip_device_info(char* host_buf, size_t len);
  * kernel time (ms), the summed algorithmic flops and the launch count.
  * Kernel classes: ACETN_HIP_K_GEMM_F64 (2*M*N*K per launch),
  * ACETN_HIP_K_GEMM_SPLIT (2*M*N*K per launch, not counting the bf16 piece
- * products), ACETN_HIP_K_DOUBLE_LAYER (4*d*D^6 per boundary block).
+ * products), ACETN_HIP_K_DOUBLE_LAYER (4*d*D^6 per boundary block),
+ * ACETN_HIP_K_JACOBI (the one-workgroup projector eigensolver, rotations and
+ * their replay; no flop count: latency-bound), ACETN_HIP_K_CHOLESKY (the
+ * one-workgroup CholeskyQR factor and inverse, q^3/2 per launch).
  * Enabling (or disabling) resets the counters. */
-enum { ACETN_HIP_K_GEMM_F64 = 0, ACETN_HIP_K_GEMM_SPLIT = 1, ACETN_HIP_K_DOUBLE_LAYER = 2 };
+enum {
+  ACETN_HIP_K_GEMM_F64 = 0,
+  ACETN_HIP_K_GEMM_SPLIT = 1,
+  ACETN_HIP_K_DOUBLE_LAYER = 2,
+  ACETN_HIP_K_JACOBI = 3,
+  ACETN_HIP_K_CHOLESKY = 4
+};
 int acetn_hip_kernel_timing(int enable);
 int acetn_hip_kernel_timing_read(int kernel, double* host_ms, double* host_flops,
                                  int64_t* host_launches);

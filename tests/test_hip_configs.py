@@ -92,12 +92,16 @@ def test_headline_left_move_matches_oracle(hip, precision):
     # projectors within their subspace, a gauge change of the new boundary
     # tensors: compare what the gauge leaves alone -- the normalised corner
     # spectra and the energy of the moved environment (device RDMs, as the
-    # oracle's contraction at this size is slow on the host).
+    # oracle's contraction at this size is slow on the host).  The projected
+    # matrix comes from FP32-class (bf16x6) products, so a singular pair
+    # s_k is resolved to about 1e-7 s_0 / s_k: with s_min / s_0 ~ 3e-4 here
+    # the tail of the kept spectrum, and with it the new corner, moves by up
+    # to a few 1e-4 of s_0.
     for s in ost:
         for k in range(4):
             sh = np.linalg.svd(h[s]['C'][k], compute_uv=False)
             so = np.linalg.svd(ost[s]['C'][k], compute_uv=False)
-            np.testing.assert_allclose(sh / sh[0], so / so[0], rtol=0, atol=1e-5, err_msg=f"{s} C{k}")
+            np.testing.assert_allclose(sh / sh[0], so / so[0], rtol=0, atol=1e-3, err_msg=f"{s} C{k}")
     e_m = device_heis_energy(hip, hst)
     e_o = device_heis_energy(hip, DevState(ost, 2, 2, D, chi, seed))
     print(f"D=6 chi=144 left move (mixed): E/site {e_m:.12f} vs FP64 oracle {e_o:.12f}")

@@ -36,6 +36,11 @@ def _declare(lib):
     sig = {
         "acetn_hip_last_error": (ctypes.c_char_p, []),
         "acetn_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, c_size]),
+        "acetn_hip_autotune": (ctypes.c_int, [ctypes.c_int]),
+        "acetn_hip_autotune_entries": (ctypes.c_int, [ctypes.POINTER(c_i64)]),
+        "acetn_hip_autotune_clear": (ctypes.c_int, []),
+        "acetn_hip_autotune_save": (ctypes.c_int, [ctypes.c_char_p]),
+        "acetn_hip_autotune_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
         "acetn_hip_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
         "acetn_hip_kernel_timing_bytes": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_dbl)]),
         "acetn_hip_kernel_timing_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_dbl),
@@ -285,6 +290,36 @@ def device_info():
     buf = ctypes.create_string_buffer(512)
     check(lib().acetn_hip_device_info(buf, 512))
     return buf.value.decode()
+
+
+AUTOTUNE_MODES = {"off": 0, "tune": 1, "table": 2}
+
+
+def autotune(mode="tune"):
+    """Tile-shape autotuner of the matrix-core products: 'off' (built-in
+    thresholds), 'tune' (time the candidates of every new shape once, keep the
+    fastest) or 'table' (use what was tuned or loaded, tune nothing new)."""
+    check(lib().acetn_hip_autotune(AUTOTUNE_MODES[mode]))
+
+
+def autotune_entries():
+    n = c_i64(0)
+    check(lib().acetn_hip_autotune_entries(ctypes.byref(n)))
+    return n.value
+
+
+def autotune_clear():
+    check(lib().acetn_hip_autotune_clear())
+
+
+def autotune_save(path):
+    check(lib().acetn_hip_autotune_save(str(path).encode()))
+
+
+def autotune_load(path):
+    n = c_i64(0)
+    check(lib().acetn_hip_autotune_load(str(path).encode(), ctypes.byref(n)))
+    return n.value
 
 
 def kernel_timing(enable):
@@ -692,9 +727,6 @@ def als_solve(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
     dev = a1r.device
     if method not in ALS_METHODS:
         raise ValueError(f"Unknown ALS method: {method}")
-    if method != "cholesky":
-        raise NotImplementedError("als_method='pinv' is not provided by the HIP engine; "
-                                  "use 'cholesky'")
     b1, b2 = a1r.clone().contiguous(), a2r.clone().contiguous()
     sweeps, cost, info = c_i64(0), c_dbl(0.0), ctypes.c_int32(0)
     ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)

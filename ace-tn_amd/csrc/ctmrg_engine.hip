@@ -839,6 +839,32 @@ extern "C" {
 
 const char* acetn_hip_last_error(void) { return g_last_error.c_str(); }
 
+int acetn_hip_autotune(int mode) {
+  return guarded([&] { autotune_set_mode(mode); });
+}
+
+int acetn_hip_autotune_entries(int64_t* host_n) {
+  return guarded([&] {
+    ACETN_REQUIRE(host_n, "autotune_entries: null output");
+    *host_n = autotune_entries();
+  });
+}
+
+int acetn_hip_autotune_clear(void) {
+  return guarded([&] { autotune_clear(); });
+}
+
+int acetn_hip_autotune_save(const char* path) {
+  return guarded([&] { autotune_save(path); });
+}
+
+int acetn_hip_autotune_load(const char* path, int64_t* host_n) {
+  return guarded([&] {
+    const int64_t n = autotune_load(path);
+    if (host_n) *host_n = n;
+  });
+}
+
 int acetn_hip_kernel_timing(int enable) {
   return guarded([&] { kernel_timing_enable(enable != 0); });
 }

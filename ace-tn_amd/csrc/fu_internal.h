@@ -50,6 +50,6 @@ size_t als_work_elems(int nD, int D, int d);
 size_t als_lds_bytes(int nD, int D, int d);
 void als_solve_device(const double* n12, const double* a12g, double* a1r, double* a2r, int nD,
                       int D, int d, int niter, double tol, double eps, double* gwork,
-                      double* out_cost, int* out_info, hipStream_t s);
+                      double* out_cost, int* out_info, hipStream_t s, int method = 0);
 
 }  // namespace acetn

@@ -461,6 +461,109 @@ __device__ __forceinline__ void wave_sync() {
 // Solve (sym(R) + eps max|R| I) X = S in place (X -> S), wave 0 doing the
 // factorisation.  Returns (block-uniform) 1 if the Cholesky factorisation
 // failed and partial-pivoting elimination was used, 2 if that found R singular.
+// ALS system through the pseudo-inverse (als_solver.py:228-230:
+// pinv(R, hermitian=True, rcond=eps) @ S): R <- (R + R^T)/2 (no shift),
+// R = V diag(lam) V^T by cyclic two-sided Jacobi with round-robin parallel
+// pairs (R in LDS, V in the global scratch), then S <- V diag(w) V^T S with
+// w_i = 1/lam_i for |lam_i| > eps * max|lam|, else 0.
+__device__ void als_pinv_solve(const AlsShared& sh, int nR, int d, double eps,
+                               double* __restrict__ V) {
+  const int tid = threadIdx.x;
+  double* R = sh.R;
+  double* S = sh.S;
+  __shared__ double rc[128], rs[128];
+  __shared__ int rp[128], rq[128];
+  __shared__ int s_rot;
+  for (int e = tid; e < nR * nR; e += ALS_NT) {
+    const int i = e / nR, j = e % nR;
+    if (i < j) {
+      const double v = 0.5 * (R[i * nR + j] + R[j * nR + i]);
+      R[i * nR + j] = v;
+      R[j * nR + i] = v;
+    }
+    V[e] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int n = nR + (nR & 1);  // padding index n - 1 (odd nR) never rotates
+  const int half = n / 2;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    for (int round = 0; round < n - 1; ++round) {
+      // circle method: index n - 1 fixed, the others rotate
+      for (int a = tid; a < half; a += ALS_NT) {
+        int p = (a == 0) ? n - 1 : (round + a) % (n - 1);
+        int q = (round - a + (n - 1)) % (n - 1);
+        if (a == 0) q = round;
+        if (p > q) { const int t = p; p = q; q = t; }
+        double c = 1.0, sn = 0.0;
+        if (q < nR) {
+          const double app = R[p * nR + p], aqq = R[q * nR + q], apq = R[p * nR + q];
+          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && apq != 0.0) {
+            const double zeta = (aqq - app) / (2.0 * apq);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sn = c * t;
+            s_rot = 1;
+          }
+        }
+        rc[a] = c; rs[a] = sn; rp[a] = p; rq[a] = q;
+      }
+      __syncthreads();
+      // rows: R[p,:], R[q,:] <- c R[p,:] - s R[q,:], s R[p,:] + c R[q,:]
+      for (int e = tid; e < half * nR; e += ALS_NT) {
+        const int a = e / nR, k = e % nR;
+        const int p = rp[a], q = rq[a];
+        if (q >= nR || rs[a] == 0.0) continue;
+        const double c = rc[a], sn = rs[a];
+        const double x = R[p * nR + k], y = R[q * nR + k];
+        R[p * nR + k] = c * x - sn * y;
+        R[q * nR + k] = sn * x + c * y;
+      }
+      __syncthreads();
+      // columns of R and V
+      for (int e = tid; e < half * nR; e += ALS_NT) {
+        const int a = e / nR, k = e % nR;
+        const int p = rp[a], q = rq[a];
+        if (q >= nR || rs[a] == 0.0) continue;
+        const double c = rc[a], sn = rs[a];
+        const double x = R[k * nR + p], y = R[k * nR + q];
+        R[k * nR + p] = c * x - sn * y;
+        R[k * nR + q] = sn * x + c * y;
+        const double vx = V[k * nR + p], vy = V[k * nR + q];
+        V[k * nR + p] = c * vx - sn * vy;
+        V[k * nR + q] = sn * vx + c * vy;
+      }
+      __syncthreads();
+    }
+    if (!s_rot) break;
+    __syncthreads();
+  }
+  // S <- V diag(w) V^T S; T = V^T S staged in R's off-diagonal-free copy
+  double mx = 0.0;
+  for (int i = tid; i < nR; i += ALS_NT) mx = fmax(mx, fabs(R[i * nR + i]));
+  mx = block_max(mx, sh.red);
+  double* T = R;  // the eigenvalues are read once below, before T overwrites row 0
+  __shared__ double lam[256];
+  for (int i = tid; i < nR; i += ALS_NT) lam[i] = R[i * nR + i];
+  __syncthreads();
+  for (int e = tid; e < nR * d; e += ALS_NT) {
+    const int i = e / d, c = e % d;
+    double acc = 0.0;
+    for (int k = 0; k < nR; ++k) acc += V[k * nR + i] * S[k * d + c];
+    const double li = lam[i];
+    T[e] = (fabs(li) > eps * mx) ? acc / li : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < nR * d; e += ALS_NT) {
+    const int i = e / d, c = e % d;
+    double acc = 0.0;
+    for (int k = 0; k < nR; ++k) acc += V[i * nR + k] * T[k * d + c];
+    S[e] = acc;
+  }
+  __syncthreads();
+}
+
 __device__ int als_solve_system(const AlsShared& sh, int nR, int d, double eps,
                                 double* __restrict__ Rsave, int* flag) {
   const int tid = threadIdx.x, lane = tid & 63;
@@ -624,7 +727,7 @@ __global__ __launch_bounds__(ALS_NT) void als_kernel(const double* __restrict__ 
                                                      double* __restrict__ a1r,
                                                      double* __restrict__ a2r, int nD, int D,
                                                      int d, int niter, double tol, double eps,
-                                                     double* __restrict__ gwork,
+                                                     int method, double* __restrict__ gwork,
                                                      double* __restrict__ out_cost,
                                                      int* __restrict__ out_info) {
   extern __shared__ double lds[];
@@ -659,12 +762,15 @@ __global__ __launch_bounds__(ALS_NT) void als_kernel(const double* __restrict__ 
   int sweeps = 0, info = 0;
   for (int i = 0; i < niter; ++i) {
     als_build(sh, n12, 0, nD, D, d, sh.a2);
-    int st = als_solve_system(sh, nR, d, eps, gwork, &flag);
+    int st = 0;
+    if (method == 1) als_pinv_solve(sh, nR, d, eps, gwork);
+    else st = als_solve_system(sh, nR, d, eps, gwork, &flag);
     info |= st;
     for (int e = tid; e < nR * d; e += ALS_NT) sh.a1[e] = sh.S[e];
     __syncthreads();
     als_build(sh, n12, 1, nD, D, d, sh.a1);
-    st = als_solve_system(sh, nR, d, eps, gwork, &flag);
+    if (method == 1) als_pinv_solve(sh, nR, d, eps, gwork);
+    else st = als_solve_system(sh, nR, d, eps, gwork, &flag);
     info |= st;
     for (int e = tid; e < nR * d; e += ALS_NT) sh.a2[e] = sh.S[e];
     __syncthreads();
@@ -776,7 +882,9 @@ size_t als_work_elems(int nD, int D, int d) {
 
 void als_solve_device(const double* n12, const double* a12g, double* a1r, double* a2r, int nD,
                       int D, int d, int niter, double tol, double eps, double* gwork,
-                      double* out_cost, int* out_info, hipStream_t s) {
+                      double* out_cost, int* out_info, hipStream_t s, int method) {
+  ACETN_REQUIRE(method == 0 || method == 1, "als: unknown method");
+  ACETN_REQUIRE(method == 0 || nD * D <= 256, "als: pinv needs nD * D <= 256");
   const size_t lds = als_lds_bytes(nD, D, d);
   ACETN_REQUIRE(lds <= 150 * 1024, "als: reduced tensors too large for one workgroup");
   static bool attr = false;
@@ -786,7 +894,7 @@ void als_solve_device(const double* n12, const double* a12g, double* a1r, double
     attr = true;
   }
   hipLaunchKernelGGL(als_kernel, dim3(1), dim3(ALS_NT), lds, s, n12, a12g, a1r, a2r, nD, D, d,
-                     niter, tol, eps, gwork, out_cost, out_info);
+                     niter, tol, eps, method, gwork, out_cost, out_info);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

@@ -526,8 +526,8 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
   return guarded([&] {
     const FuDims f = fu_dims(D, d);
     ACETN_REQUIRE(n12 && a12g && a1r && a2r, "als_solve: null pointer");
-    ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY,
-                  "als_solve: only the 'cholesky' method (with its LU fallback) runs on the device");
+    ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY || method == ACETN_HIP_ALS_PINV,
+                  "als_solve: unknown method");
     ACETN_REQUIRE(niter >= 0, "als_solve: negative niter");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       double* gw = c.take(als_work_elems((int)f.nD, (int)f.D, (int)f.d));
@@ -535,7 +535,7 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
       int* info = c.take_int(4);
       if (!c.launch) return;
       als_solve_device(n12, a12g, a1r, a2r, (int)f.nD, (int)f.D, (int)f.d, niter, tol, epsilon,
-                       gw, cost, info, c.st);
+                       gw, cost, info, c.st, method == ACETN_HIP_ALS_PINV ? 1 : 0);
       double hc = 0.0;
       int hi[2] = {0, 0};
       ACETN_HIP_CHECK(hipMemcpyAsync(&hc, cost, sizeof(double), hipMemcpyDeviceToHost, c.st));

@@ -399,8 +399,7 @@ Tile choose_tile(const GemmDesc& g) {
   return Tile{2, 2};
 }
 
-int choose_ksplit(const GemmDesc& g, int count) {
-  const Tile t = choose_tile(g);
+int choose_ksplit_for(const GemmDesc& g, int count, const Tile& t) {
   const int BM = 32 * t.wm, BN = 32 * t.wn;
   const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * count;
   if (tiles >= 512 || g.K < 4 * BK * 8) return 1;
@@ -408,6 +407,10 @@ int choose_ksplit(const GemmDesc& g, int count) {
   ks = std::min(ks, g.K / (BK * 8));
   ks = std::min(ks, 64);
   return std::max(ks, 1);
+}
+
+int choose_ksplit(const GemmDesc& g, int count) {
+  return choose_ksplit_for(g, count, choose_tile(g));
 }
 
 template <int WM, int WN>
@@ -448,24 +451,51 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
     ACETN_HIP_CHECK(hipGetLastError());
     return;
   }
-  int ks = choose_ksplit(g, b.count);
-  if (ks > 1 && (work == nullptr || work_elems < (size_t)ks * b.count * g.M * g.N)) ks = 1;
-  const Tile t = choose_tile(g);
-  hipEvent_t e0 = kernel_timing_begin(s);
-  if (t.wm == 2 && t.wn == 5) launch_gemm<2, 5>(g, b, ks, work, s);
-  else if (t.wm == 1 && t.wn == 5) launch_gemm<1, 5>(g, b, ks, work, s);
-  else if (t.wm == 4 && t.wn == 4) launch_gemm<4, 4>(g, b, ks, work, s);
-  else launch_gemm<2, 2>(g, b, ks, work, s);
-  ACETN_HIP_CHECK(hipGetLastError());
-  kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count,
-                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N) * b.count);
-  if (ks > 1) {
-    const int64_t mn = (int64_t)g.M * g.N;
-    int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks, b.count), dim3(256), 0, s, g, b, ks,
-                       work);
+  auto fits = [&](int k) {
+    return k == 1 || (work != nullptr && work_elems >= (size_t)k * b.count * g.M * g.N);
+  };
+  auto launch_choice = [&](const TuneChoice& c) {
+    hipEvent_t e0 = kernel_timing_begin(s);
+    if (c.wm == 2 && c.wn == 5) launch_gemm<2, 5>(g, b, c.ks, work, s);
+    else if (c.wm == 1 && c.wn == 5) launch_gemm<1, 5>(g, b, c.ks, work, s);
+    else if (c.wm == 4 && c.wn == 4) launch_gemm<4, 4>(g, b, c.ks, work, s);
+    else launch_gemm<2, 2>(g, b, c.ks, work, s);
     ACETN_HIP_CHECK(hipGetLastError());
+    kernel_timing_end(e0, s, kKGemmF64, 2.0 * g.M * g.N * (double)g.K * b.count,
+                      8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N) * b.count);
+    if (c.ks > 1) {
+      const int64_t mn = (int64_t)g.M * g.N;
+      int blocks = (int)std::min<int64_t>((mn + 255) / 256, 2048);
+      hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks, b.count), dim3(256), 0, s, g, b, c.ks,
+                         work);
+      ACETN_HIP_CHECK(hipGetLastError());
+    }
+  };
+  const bool lin = g.am.n0 <= 0 && g.ak.n0 <= 0 && g.bk.n0 <= 0 && g.bn.n0 <= 0;
+  const TuneKey key{0, g.M, g.N, g.K, (lin ? 1 : 0) | (b.count << 1)};
+  if (autotune_should_tune(key, g, s)) {
+    std::vector<TuneChoice> cands;
+    const Tile tiles[4] = {{1, 5}, {2, 5}, {2, 2}, {4, 4}};
+    for (const Tile& t : tiles) {
+      const int h = choose_ksplit_for(g, b.count, t);
+      const int opts[4] = {1, h, 2 * h, h / 2};
+      for (int k : opts) {
+        if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < BK)) continue;
+        bool dup = false;
+        for (const TuneChoice& c : cands) dup |= c.wm == t.wm && c.wn == t.wn && c.ks == k;
+        if (!dup) cands.push_back(TuneChoice{t.wm, t.wn, k});
+      }
+    }
+    autotune_run(key, cands, s, launch_choice);
+    return;
   }
+  int ks = choose_ksplit(g, b.count);
+  if (!fits(ks)) ks = 1;
+  const Tile t = choose_tile(g);
+  TuneChoice ch{t.wm, t.wn, ks};
+  TuneChoice tuned;
+  if (autotune_lookup(key, &tuned) && fits(tuned.ks)) ch = tuned;
+  launch_choice(ch);
 }
 
 GemmDesc dense_desc(int M, int N, int K, bool ta, int64_t lda, bool tb, int64_t ldb,

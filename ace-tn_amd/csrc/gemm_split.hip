@@ -664,8 +664,7 @@ int planar_wgs_per_cu(const Tile& t) {
 // resident workgroups on the 256 CUs (a second, partly filled round costs as
 // much as a full one), each part at least 4 k-tiles long.  per_cu = 0 gives
 // the upper bound used to size the scratch (LDS limit only).
-int choose_ksplit(const GemmDesc& g, int split, int per_cu) {
-  const Tile t = choose_tile(g);
+int choose_ksplit_for(const GemmDesc& g, int split, int per_cu, const Tile& t) {
   const int BM = 32 * t.wm, BN = 32 * t.wn;
   const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
   if (per_cu <= 0) {
@@ -678,6 +677,9 @@ int choose_ksplit(const GemmDesc& g, int split, int per_cu) {
   ks = std::min(ks, g.K / (BK * 4));
   ks = std::min(ks, 64);
   return std::max(ks, 1);
+}
+int choose_ksplit(const GemmDesc& g, int split, int per_cu) {
+  return choose_ksplit_for(g, split, per_cu, choose_tile(g));
 }
 int choose_ksplit_run(const GemmDesc& g, int split) {
   const Tile t = choose_tile(g);
@@ -736,26 +738,60 @@ void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* B
 // C = A B with A given as planes; B (FP64 through its index maps) is split
 // into planes after the partials in the scratch, unless Bin already holds
 // them.  With Cp the result goes to planes (see gemm_split_planar).
+// ``work_elems`` (when nonzero) lets the autotuner try other tiles and split
+// counts that fit the scratch; the caller's ks is the threshold choice.
 template <int SPLIT>
 void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, double* C, int ks,
-         double* work, hipStream_t s, const SplitPlanes* Bin = nullptr, SplitPlanes* Cp = nullptr) {
+         double* work, hipStream_t s, const SplitPlanes* Bin = nullptr, SplitPlanes* Cp = nullptr,
+         size_t work_elems = 0, int flags = 0) {
   const int Kp = padded_k(g.K);
   ACETN_REQUIRE((int64_t)SPLIT * aplane < (int64_t(1) << 31) &&
                     (int64_t)SPLIT * g.N * Kp < (int64_t(1) << 31),
                 "gemm_split: operand planes exceed 2^31 elements");
-  __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, ks));
-  if (Bin != nullptr) {
+  if (Bin != nullptr)
     ACETN_REQUIRE(Bin->p && Bin->rows == g.N && Bin->kp == Kp && Bin->pieces >= SPLIT,
                   "gemm_split: B planes do not match the product");
-    planes = reinterpret_cast<__bf16*>(Bin->p);
-  } else {
-    split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
+  // the B planes follow the split-K partials in the scratch, so their place
+  // depends on the split count; each launch splits B again
+  auto launch_choice = [&](const TuneChoice& c) {
+    __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, c.ks));
+    if (Bin != nullptr) planes = reinterpret_cast<__bf16*>(Bin->p);
+    else split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
+    if (c.wm == 2 && c.wn == 5) launch<2, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+    else if (c.wm == 1 && c.wn == 5) launch<1, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+    else if (c.wm == 4 && c.wn == 4) launch<4, 4, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+    else launch<2, 2, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+  };
+  const Tile t0 = choose_tile(g);
+  TuneChoice ch{t0.wm, t0.wn, ks};
+  // products that write C planes keep the threshold choice (their split-K
+  // reduction writes the planes, so the split count changes the kernels run)
+  if (Cp == nullptr && work_elems > 0) {
+    auto fits = [&](int k) {
+      return partial_elems(g, k) + (Bin ? 0 : plane_elems(g, SPLIT)) <= work_elems;
+    };
+    const TuneKey key{SPLIT, g.M, g.N, g.K,
+                      flags | (Bin ? 4 : 0) | (g.c_f32 ? 8 : 0) | (g.a_f32 ? 16 : 0)};
+    if (autotune_should_tune(key, g, s)) {
+      std::vector<TuneChoice> cands;
+      const Tile tiles[4] = {{1, 5}, {2, 5}, {2, 2}, {4, 4}};
+      for (const Tile& t : tiles) {
+        const int h = choose_ksplit_for(g, SPLIT, planar_wgs_per_cu<SPLIT>(t), t);
+        const int opts[4] = {1, h, 2 * h, h / 2};
+        for (int k : opts) {
+          if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < BK)) continue;
+          bool dup = false;
+          for (const TuneChoice& c : cands) dup |= c.wm == t.wm && c.wn == t.wn && c.ks == k;
+          if (!dup) cands.push_back(TuneChoice{t.wm, t.wn, k});
+        }
+      }
+      autotune_run(key, cands, s, launch_choice);
+      return;
+    }
+    TuneChoice tuned;
+    if (autotune_lookup(key, &tuned) && fits(tuned.ks)) ch = tuned;
   }
-  const Tile t = choose_tile(g);
-  if (t.wm == 2 && t.wn == 5) launch<2, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
-  else if (t.wm == 1 && t.wn == 5) launch<1, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
-  else if (t.wm == 4 && t.wn == 4) launch<4, 4, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
-  else launch<2, 2, SPLIT>(g, Ap, aplane, planes, Kp, C, ks, work, s, Cp);
+  launch_choice(ch);
 }
 }  // namespace
 
@@ -796,8 +832,10 @@ void gemm_split(const GemmDesc& g, const GemmBatch& b, int split, DType ta, DTyp
     // planes[s][m][k] of A(m, k) = A[am(m) + ak(k)]: the B-plane split with the maps swapped
     if (split == 2) split_into_planes<2>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s, g.a_f32);
     else split_into_planes<3>(b.A[0], g.ak, g.am, g.K, g.M, Kp, Ap, s, g.a_f32);
-    if (split == 2) run<2>(g, Ap, total, b.B[0], b.C[0], ks, work, s);
-    else run<3>(g, Ap, total, b.B[0], b.C[0], ks, work, s);
+    // the tuner may use the scratch up to the A planes
+    const size_t avail = partial_elems(g, ks) + plane_elems(g, 3);
+    if (split == 2) run<2>(g, Ap, total, b.B[0], b.C[0], ks, work, s, nullptr, nullptr, avail, 1);
+    else run<3>(g, Ap, total, b.B[0], b.C[0], ks, work, s, nullptr, nullptr, avail, 1);
     return;
   }
   ACETN_REQUIRE(!g.a_f32, "gemm_split: an FP32 A operand needs the plane path");
@@ -838,8 +876,8 @@ void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B,
   }
   const int64_t aplane = (int64_t)A.rows * A.kp;
   const __bf16* Ap = reinterpret_cast<const __bf16*>(A.p);
-  if (split == 2) run<2>(g, Ap, aplane, B, C, ks, work, s, Bp, Cp);
-  else run<3>(g, Ap, aplane, B, C, ks, work, s, Bp, Cp);
+  if (split == 2) run<2>(g, Ap, aplane, B, C, ks, work, s, Bp, Cp, work_elems, 2);
+  else run<3>(g, Ap, aplane, B, C, ks, work, s, Bp, Cp, work_elems, 2);
 }
 
 int split_planes_kp(int K) { return padded_k(K); }

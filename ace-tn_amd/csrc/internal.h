@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace acetn {
 
@@ -133,6 +135,31 @@ void split_transpose(const double* X, int R, int Cc, int64_t ld, SplitPlanes& N,
 void gemm_split_planar(const GemmDesc& g, const SplitPlanes& A, const double* B, double* C,
                        int split, double* work, size_t work_elems, hipStream_t s,
                        const SplitPlanes* Bp = nullptr, SplitPlanes* Cp = nullptr);
+
+// Tile-shape autotuner (autotune.hip).  kind: 0 FP64 product, 2 / 3 split
+// product with that many bf16 pieces; flags: operand-layout bits of the
+// caller.  A choice is a tile (wm x wn MFMA blocks per wave) and a split-K
+// count; ms is its measured time.
+struct TuneKey {
+  int kind, M, N, K, flags;
+};
+struct TuneChoice {
+  int wm, wn, ks;
+  double ms = 0.0;
+};
+void autotune_set_mode(int mode);  // 0 thresholds, 1 tune new shapes, 2 table only
+int autotune_mode();
+bool autotune_lookup(const TuneKey& k, TuneChoice* out);
+// whether this product should be tuned now (mode 1, shape not in the table,
+// beta == 0, stream not capturing)
+bool autotune_should_tune(const TuneKey& k, const GemmDesc& g, hipStream_t s);
+// times every candidate through ``launch`` on s, stores and re-runs the fastest
+TuneChoice autotune_run(const TuneKey& k, const std::vector<TuneChoice>& cands, hipStream_t s,
+                        const std::function<void(const TuneChoice&)>& launch);
+int64_t autotune_entries();
+void autotune_clear();
+void autotune_save(const char* path);
+int64_t autotune_load(const char* path);
 
 // Live timing of the matrix-core kernels (HIP events on the launch stream),
 // per kernel class; the benchmark's roofline figures come from here.

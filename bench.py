@@ -206,6 +206,22 @@ def run_ctmrg(precision, args, A, keep_state=False):
         ctmrg_iteration(mover, net)
         log(f"[{precision}] warmup {i}: C0 {tuple(net[(0, 0)]['C'][0].shape)}")
     torch.cuda.synchronize()
+    mode = args.autotune if precision == "fp64" else args.autotune_mixed
+    if mode != "off":
+        # one more iteration tunes the tile shape of each product shape (with
+        # the projector pairs serialised, every product timed alone, or
+        # concurrent as in the timed loop); the timed iterations then use the
+        # table
+        _hip.autotune_clear()
+        _hip.autotune("tune")
+        _hip.serialize(mode == "serial")
+        t0 = time.perf_counter()
+        ctmrg_iteration(mover, net)
+        torch.cuda.synchronize()
+        _hip.serialize(False)
+        _hip.autotune("table")
+        log(f"[{precision}] autotuned {_hip.autotune_entries()} product shapes in "
+            f"{time.perf_counter() - t0:.1f} s")
     warm = None
     if keep_state:
         warm = {s: {'A': net[s]['A'].cpu().numpy(), 'C': [c.cpu().numpy() for c in net[s]['C']],
@@ -234,6 +250,8 @@ def run_ctmrg(precision, args, A, keep_state=False):
             stats[name] = {"ms": ms, "flops": fl, "launches": n,
                            "bytes": _hip.kernel_timing_bytes(cls)}
     _hip.kernel_timing(False)
+    for _ in range(args.energy_iters):
+        ctmrg_iteration(mover, net)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     energy, corr = bond_observables(net)
@@ -452,6 +470,14 @@ def main():
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 12-instance parameter sweep")
     ap.add_argument("--sweep-steps", type=int, default=3)
+    ap.add_argument("--autotune", choices=["off", "serial", "concurrent"], default="serial",
+                    help="tile autotuning of the FP64 run: off (built-in thresholds), or one "
+                         "tuning iteration with the projector pairs serialised / concurrent")
+    ap.add_argument("--autotune-mixed", choices=["off", "serial", "concurrent"], default="serial",
+                    help="the same for the mixed-precision run")
+    ap.add_argument("--energy-iters", type=int, default=0,
+                    help="untimed iterations after the timed ones, before the energies are "
+                         "measured")
     ap.add_argument("--serial", action="store_true",
                     help="run the timed iterations with the projector pairs serialised")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
@@ -488,6 +514,14 @@ def main():
     ref = run_ctmrg("fp64", args, A, keep_state=not args.no_cpu_baseline)
     mix = run_ctmrg("mixed", args, A)
     dE = abs(mix["energy"] - ref["energy"])
+    # the FP64 run's own rounding spread: the same FP64 run with the other
+    # tile choice (built-in thresholds, or tuned), i.e. the same arithmetic
+    # summed in another order
+    import copy as _copy
+    alt = _copy.copy(args)
+    alt.autotune = "off" if args.autotune != "off" else "serial"
+    ref2 = run_ctmrg("fp64", alt, A)
+    spread = abs(ref2["energy"] - ref["energy"])
     # Roofline of the matrix-core class with the most event time in the
     # headline (mixed) run.  The split-bf16 class is HBM-bound by
     # construction (its MFMA work is 3 or 6 bf16 piece products per
@@ -539,6 +573,8 @@ def main():
         "bond_rdm_measure_ms": 1e3 * ref["measure_s"],
         "energy_per_site": {"mixed": mix["energy"], "fp64": ref["energy"], "abs_diff": dE,
                             "rel_diff": dE / abs(ref["energy"]),
+                            "fp64_rounding_spread": spread,
+                            "fp64_other_tiles": ref2["energy"],
                             "note": "Heisenberg J=1 bond operator on the final environments; "
                                     "the FP64 HIP path is held to the FP64 oracle by the tests"},
         "nn_correlators": {"mixed": mix["correlators"], "fp64": ref["correlators"],
@@ -557,6 +593,7 @@ def main():
                      "avg_launch_ms": st["ms"] / st["launches"]},
         "dominant_kernel_by_time": dominant,
         "kernel_time_shares": shares,
+        "autotune": {"fp64": args.autotune, "mixed": args.autotune_mixed},
         "kernels": {"mixed": mix["stats"], "fp64": ref["stats"]},
     }
     if not args.no_sweep:

@@ -61,6 +61,20 @@ enum { ACETN_HIP_FP64 = 0, ACETN_HIP_MIXED = 1 };
 /* Message of the last failed call on this thread ("" if none). */
 const char* acetn_hip_last_error(void);
 
+/* Tile-shape autotuner of the matrix-core products (no reference
+ * counterpart).  Mode 0: the built-in thresholds pick each product's tile and
+ * split-K count; 1: the first product of every new shape times the candidate
+ * tiles and split counts on the caller's stream (HIP events, a host wait per
+ * tuned shape) and the fastest is kept in a per-process table that later
+ * products use (shapes under stream capture and accumulating products are
+ * not tuned); 2: the table is used, nothing new is tuned.  _save/_load write
+ * and read the table as text (one shape per line). */
+int acetn_hip_autotune(int mode);
+int acetn_hip_autotune_entries(int64_t* host_n);
+int acetn_hip_autotune_clear(void);
+int acetn_hip_autotune_save(const char* path);
+int acetn_hip_autotune_load(const char* path, int64_t* host_n);
+
 /* Library / device self-description: writes up to ``len`` bytes. */
 int acetn_hip_device_info(char* host_buf, size_t len);
 

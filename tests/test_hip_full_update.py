@@ -166,6 +166,22 @@ def test_als_solve_matches_oracle(hip, bond_idx, tol):
                                ein("yup,xuq->yxpq", r1, r2), atol=1e-9 * np.abs(a12g).max())
 
 
+@pytest.mark.parametrize("bond_idx", [0, 5])
+def test_als_pinv_matches_oracle(hip, bond_idx):
+    """method='pinv' (als_solver.py:228-230): the pseudo-inverse solve of the
+    symmetrised system with relative cutoff epsilon, sweep for sweep."""
+    n12, a12g, shape = als_problem(bond_idx)
+    start = ofu.initialize_reduced_tensors(a12g, shape)
+    r1, r2, rs, rc = ofu.als_solve(n12, a12g, shape, 100, 1e-12, "pinv", 1e-12, start=start)
+    h1, h2, hs, hc, flags = hip.als_solve(dev(n12), dev(a12g), dev(start[0]), dev(start[1]), 100,
+                                          1e-12, "pinv", 1e-12)
+    print(f"pinv bond {bond_idx}: sweeps oracle {rs} hip {hs}, cost {rc:.15e} {hc:.15e}")
+    assert hs == rs
+    assert hc == pytest.approx(rc, rel=1e-10)
+    np.testing.assert_allclose(ein("yup,xuq->yxpq", host(h1), host(h2)),
+                               ein("yup,xuq->yxpq", r1, r2), atol=1e-8 * np.abs(a12g).max())
+
+
 def test_als_zero_distance_for_exact_product(hip):
     """Closed form: a12g = a1r a2r exactly (rank <= D): the optimum cost is
     -<a12g|N|a12g>."""

@@ -53,7 +53,14 @@ class TensorNetwork:
         torch.save(state, filename)
 
     def load(self, filename):
-        """tensor_network.py:111-122 for the plain-tensor format of save()."""
+        """tensor_network.py:111-122: reads the engine's plain-tensor format of
+        save() (torch.load with weights_only=True), or a checkpoint written by
+        the reference's save(), whose pickled acetn objects are read without
+        executing the pickle (acetn/ipeps/checkpoint.py)."""
+        from .checkpoint import is_reference_checkpoint, read_reference_checkpoint
+        if is_reference_checkpoint(filename):
+            self.load_arrays(read_reference_checkpoint(filename, self.nx, self.ny))
+            return
         state = torch.load(filename, weights_only=True)
         for key, t in state["tensors"].items():
             site = tuple(int(v) for v in key.split(","))

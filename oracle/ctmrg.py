@@ -14,6 +14,7 @@ observable unchanged.
 """
 import numpy as np
 
+from . import linalg
 from .rng import SketchStream
 
 ein = lambda s, *ops: np.einsum(s, *ops, optimize=True)
@@ -106,11 +107,11 @@ def fused_matmul_svd_lowrank(A, B, q, niter, omega_fn):
     Om = omega_fn(n, q)
     Y = A @ (B @ Om)
     for _ in range(niter):
-        Y, _ = np.linalg.qr(Y)
+        Y, _ = linalg.qr(Y)
         Y = A @ (B @ (B.conj().T @ (A.conj().T @ Y)))
-    Qm, _ = np.linalg.qr(Y)
+    Qm, _ = linalg.qr(Y)
     Bp = (Qm.conj().T @ A) @ B
-    Ub, S, Vh = np.linalg.svd(Bp, full_matrices=False)
+    Ub, S, Vh = linalg.svd(Bp, full_matrices=False)
     return Qm @ Ub, S, Vh.conj().T
 
 
@@ -122,13 +123,13 @@ def fused_3matmul_svd_lowrank(A, B, C, D, q, niter, omega_fn):
     H = lambda X: X.conj().T
     Y = A @ (B @ (C @ (D @ Om)))
     for _ in range(niter):
-        Y, _ = np.linalg.qr(Y)
+        Y, _ = linalg.qr(Y)
         Y = H(D) @ (H(C) @ (H(B) @ (H(A) @ Y)))
-        Y, _ = np.linalg.qr(Y)
+        Y, _ = linalg.qr(Y)
         Y = A @ (B @ (C @ (D @ Y)))
-    Qm, _ = np.linalg.qr(Y)
+    Qm, _ = linalg.qr(Y)
     Bp = (((H(Qm) @ A) @ B) @ C) @ D
-    Ub, S, Vh = np.linalg.svd(Bp, full_matrices=False)
+    Ub, S, Vh = linalg.svd(Bp, full_matrices=False)
     return Qm @ Ub, S, H(Vh)
 
 
@@ -139,10 +140,10 @@ def svd_lowrank(A, q, niter, omega_fn):
     Om = omega_fn(n, q)
     Y = A @ Om
     for _ in range(niter):
-        Y, _ = np.linalg.qr(Y)
+        Y, _ = linalg.qr(Y)
         Y = A @ (A.conj().T @ Y)
-    Qm, _ = np.linalg.qr(Y)
-    Ub, S, Vh = np.linalg.svd(Qm.conj().T @ A, full_matrices=False)
+    Qm, _ = linalg.qr(Y)
+    Ub, S, Vh = linalg.svd(Qm.conj().T @ A, full_matrices=False)
     return Qm @ Ub, S, Vh.conj().T
 
 
@@ -158,6 +159,8 @@ def fix_projector_gauge(P1, P2):
     """Sign convention shared with the device engine: for every retained
     singular index z, sign(sum_i w_i P1[i, z]) is made positive and P2[:, z]
     follows."""
+    if not linalg.state.gauges:
+        return P1, P2
     n = P1.shape[-1]
     f1 = P1.reshape(-1, n)
     s = gauge_weights(f1.shape[0]) @ f1
@@ -205,7 +208,7 @@ class ProjectorCalculator:
         if self.svd_type == "full-rank":
             R = Q1 @ Q4
             R = R / np.abs(R).max()
-            Uf, s, Vh = np.linalg.svd(R)
+            Uf, s, Vh = linalg.svd(R)
             U, V = Uf, Vh.conj().T
         else:
             U, s, V = fused_matmul_svd_lowrank(Q1, Q4, chi + self.rsvd_oversampling,
@@ -231,7 +234,7 @@ class ProjectorCalculator:
             R2 = R2 / np.abs(R2).max()
             F = R1 @ R2
             F = F / np.abs(F).max()
-            Uf, s, Vh = np.linalg.svd(F)
+            Uf, s, Vh = linalg.svd(F)
             U, V, cD_new = self._truncate(Uf, s, Vh.conj().T, chi)
             P1 = ein("xedD,xz->edDz", R1.reshape(R1.shape[0], *q1D[3:]), U.conj())
             P2 = ein("cuUy,yz->cuUz", R2.reshape(*q4D[:3], R2.shape[1]), V)

@@ -13,6 +13,8 @@ observable unchanged.
 import numpy as np
 import scipy.linalg
 
+from . import linalg
+
 from .ctmrg import ein, bond_permute, DirectionalMover
 from .measure import bond_env
 
@@ -25,8 +27,8 @@ def decompose_site_tensors(a1, a2):
     """tensor_update.py:52-68."""
     bD, pD = a1.shape[3:]
     nD = min(bD ** 3, pD * bD)
-    q1, r1 = np.linalg.qr(ein("lurdp->rdulp", a1).reshape(bD ** 3, pD * bD))
-    q2, r2 = np.linalg.qr(ein("lurdp->dlurp", a2).reshape(bD ** 3, pD * bD))
+    q1, r1 = linalg.qr(ein("lurdp->rdulp", a1).reshape(bD ** 3, pD * bD))
+    q2, r2 = linalg.qr(ein("lurdp->dlurp", a2).reshape(bD ** 3, pD * bD))
     return (q1.reshape(bD, bD, bD, nD), r1.reshape(nD, bD, pD),
             q2.reshape(bD, bD, bD, nD), r2.reshape(nD, bD, pD))
 
@@ -66,9 +68,9 @@ def positive_approx(n12, cutoff=1e-12):
 
     def _eig(M):
         try:
-            return np.linalg.eigh(M)
+            return linalg.eigh(M)
         except np.linalg.LinAlgError:
-            u, w, _ = np.linalg.svd(M)
+            u, w, _ = linalg.svd(M)
             return w[::-1].copy(), u[:, ::-1].copy()
 
     nw, nz = _eig(N)
@@ -81,7 +83,7 @@ def positive_approx(n12, cutoff=1e-12):
 def pinv_atol(a, atol):
     """torch.linalg.pinv(a, atol=atol) with rtol=0 (singular values <= atol
     are dropped)."""
-    u, s, vh = np.linalg.svd(a)
+    u, s, vh = linalg.svd(a)
     keep = s > atol
     sinv = np.zeros_like(s)
     sinv[keep] = 1.0 / s[keep]
@@ -91,8 +93,8 @@ def pinv_atol(a, atol):
 def gauge_fix(nz, a12g, atol=1e-12):
     """full_update.py:300-343."""
     nD = a12g.shape[0]
-    _, nzyr = np.linalg.qr(ein("yxz->zxy", nz).reshape(nD ** 3, nD))
-    _, nzxr = np.linalg.qr(ein("yxz->zyx", nz).reshape(nD ** 3, nD))
+    _, nzyr = linalg.qr(ein("yxz->zxy", nz).reshape(nD ** 3, nD))
+    _, nzxr = linalg.qr(ein("yxz->zyx", nz).reshape(nD ** 3, nD))
     nzyr_inv = pinv_atol(nzyr, atol)
     nzxr_inv = pinv_atol(nzxr, atol)
     nz = ein("yxz,xw->yzw", nz, nzxr_inv)
@@ -110,7 +112,7 @@ def initialize_reduced_tensors(a12g, ar_shape):
     """als_solver.py:117-146 (the 'plain start')."""
     nD, bD, pD = ar_shape
     m = ein("yxpq->ypxq", a12g).reshape(nD * pD, nD * pD)
-    U, S, Vh = np.linalg.svd(m)
+    U, S, Vh = linalg.svd(m)
     V = Vh.conj().T
     S = np.sqrt(S[:bD] / S[0])
     U = U[:, :bD].reshape(nD, pD, bD)
@@ -159,7 +161,7 @@ def solve_ar(R, S, method="cholesky", epsilon=1e-12):
         except np.linalg.LinAlgError:
             ar = np.linalg.solve(R, S)
     elif method == "pinv":
-        u, s, vh = np.linalg.svd(R)
+        u, s, vh = linalg.svd(R)
         keep = s > epsilon * s[0]
         sinv = np.where(keep, 1.0 / np.where(keep, s, 1.0), 0.0)
         ar = (vh.conj().T * sinv) @ (u.conj().T @ S)
@@ -195,6 +197,8 @@ def fix_bond_gauge(a1r, a2r):
     sign(sum_i w_i a1r[i, u]) > 0 over the flattened (y, p) entries of each
     bond index u, with the weights of oracle.ctmrg.gauge_weights."""
     from .ctmrg import gauge_weights
+    if not linalg.state.gauges:
+        return a1r, a2r
     nD, bD, pD = a1r.shape
     f = np.transpose(a1r, (1, 0, 2)).reshape(bD, -1)
     s = f @ gauge_weights(f.shape[1])
@@ -208,9 +212,9 @@ def finalize_reduced_tensors(a1r, a2r, nzyr_inv=None, nzxr_inv=None):
         a1r = ein("yz,zup->yup", nzyr_inv, a1r)
         a2r = ein("xw,wvq->xvq", nzxr_inv, a2r)
     nD, bD, pD = a1r.shape
-    q1, r1 = np.linalg.qr(ein("yup->ypu", a1r).reshape(nD * pD, bD))
-    q2, r2 = np.linalg.qr(ein("xvq->xqv", a2r).reshape(nD * pD, bD))
-    U, s, Vh = np.linalg.svd(ein("au,bu->ab", r1, r2))
+    q1, r1 = linalg.qr(ein("yup->ypu", a1r).reshape(nD * pD, bD))
+    q2, r2 = linalg.qr(ein("xvq->xqv", a2r).reshape(nD * pD, bD))
+    U, s, Vh = linalg.svd(ein("au,bu->ab", r1, r2))
     s = np.sqrt(s[:bD] / np.linalg.norm(s))
     r1 = ein('ab,b->ab', U[:, :bD], s)
     r2 = ein('ba,b->ba', Vh[:bD, :], s)
@@ -330,7 +334,7 @@ def simple_update_bond(st, bond, gate, lam):
     nD, bD, pD = a1r.shape
     a12 = ein("yup,xuq->ypxq", a1r, a2r)
     a12g = ein("ypxq,pqrs->yrxs", a12, gate)
-    U, S, Vh = np.linalg.svd(a12g.reshape(nD * pD, nD * pD))
+    U, S, Vh = linalg.svd(a12g.reshape(nD * pD, nD * pD))
     S = S[:bD]
     lt = np.sqrt(S / np.linalg.norm(S))
     U = U[:, :bD].reshape(nD, pD, bD)

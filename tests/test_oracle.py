@@ -256,3 +256,100 @@ def test_bf16_rounding_known_values():
     assert omlp.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6   # tie -> even (up)
     assert omlp.bf16(1.0 + 2 ** -7) == 1.0 + 2 ** -7
     assert omlp.bf16(-3.0) == -3.0
+
+
+# ------------------------------------------------------ reference trajectory
+def load_reference_trajectory(name):
+    import os
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "reference_trajectory.npz"))
+    D, chi, aseed, tseed, steps, dtau, nsteps = f[f"{name}/meta"]
+
+    def state(tag):
+        st = {}
+        for s in [(0, 0), (0, 1), (1, 0), (1, 1)]:
+            p = f"{name}/{tag}/{s[0]}{s[1]}_"
+            st[s] = {'A': f[p + 'A'], 'C': [f[p + f'C{k}'] for k in range(4)],
+                     'E': [f[p + f'E{k}'] for k in range(4)]}
+        return st
+    return (state("init"), state("final"), float(f[f"{name}/energy"]), int(D), int(chi),
+            int(tseed), int(steps), float(dtau), int(nsteps))
+
+
+class TorchOmega:
+    """The reference's sketch: torch.randn(n, q) from torch's global generator,
+    one call per projector pair in call order (fused_matmul_svd_lowrank.py:32)."""
+
+    def __init__(self, seed):
+        import torch
+        torch.manual_seed(seed)
+
+    def __call__(self, n, q):
+        import torch
+        return torch.randn(n, q, dtype=torch.float64).numpy()
+
+
+def _corner_spectra_close(a, b, atol):
+    for s in a:
+        for k in range(4):
+            sa = np.linalg.svd(a[s]['C'][k], compute_uv=False)
+            sb = np.linalg.svd(b[s]['C'][k], compute_uv=False)
+            np.testing.assert_allclose(sa / sa[0], sb / sb[0], rtol=0, atol=atol)
+
+
+def _replay(name, init, chi, tseed, steps, dtau, nsteps, scale=0.0):
+    """The oracle on a recorded reference run: same start (optionally moved by
+    ``scale`` relative noise), torch.randn sketches in call order, torch.linalg
+    factorisations and no sign gauges (the reference has none)."""
+    from oracle import linalg
+    A = {s: v['A'] for s, v in init.items()}
+    if scale:
+        rng = np.random.default_rng(5)
+        A = {s: a * (1.0 + scale * rng.standard_normal(a.shape)) for s, a in A.items()}
+    with linalg.reference_conventions():
+        st = oc.make_state(A)
+        calc = oc.ProjectorCalculator(omega_fn=TorchOmega(tseed))
+        mover = oc.DirectionalMover(calc, 2, 2, chi)
+        for _ in range(steps):
+            oc.ctmrg_iteration(mover, st)
+        if nsteps:
+            ofu.evolve_full(st, 2, 2, omod.Heisenberg(1.0), dtau, nsteps, ofu.FullUpdateConfig(),
+                            mover, oc.bond_list(2, 2))
+            for _ in range(steps):
+                oc.ctmrg_iteration(mover, st)
+        return st, om.measure(st, 2, 2, omod.Heisenberg(1.0))['Energy']
+
+
+@pytest.mark.parametrize("name", ["ctmrg_d2_chi8", "ctmrg_d3_chi12", "evolve_d2_chi8",
+                                  "evolve_d2_chi20_100"])
+def test_oracle_replays_reference_trajectory(name):
+    """The reference run itself (tests/golden/make_reference_trajectory.py:
+    torch.randn sketches, torch.linalg factorisations): the oracle's C/E
+    initialisation equals the reference's, and the oracle on the same
+    sketches with the same factorisation conventions reaches the reference's
+    energy within 1e-10 relative, or within the spread the oracle shows when
+    its own input moves by one part in 1e15 where that is larger (a trajectory
+    that amplifies rounding: at D=2 the corners and the 100-step evolution
+    move by 1e-7 to 1e-6 under such a change, so no implementation that sums
+    in another order than torch can be held to 1e-10 there); where the
+    trajectory does not amplify rounding the final boundary tensors agree to
+    1e-10 as well."""
+    init, final, e_ref, D, chi, tseed, steps, dtau, nsteps = load_reference_trajectory(name)
+    st0 = oc.make_state({s: v['A'] for s, v in init.items()})
+    for s in st0:
+        for k in range(4):
+            np.testing.assert_allclose(st0[s]['C'][k], init[s]['C'][k], rtol=0, atol=1e-15)
+            np.testing.assert_allclose(st0[s]['E'][k], init[s]['E'][k], rtol=0, atol=1e-15)
+    st, e = _replay(name, init, chi, tseed, steps, dtau, nsteps)
+    rel = abs(e - e_ref) / abs(e_ref)
+    bound = 1e-10
+    if rel > bound:
+        _, e_p = _replay(name, init, chi, tseed, steps, dtau, nsteps, scale=1e-15)
+        bound = max(bound, 10.0 * abs(e_p - e) / abs(e))
+    print(f"{name}: oracle {e:.15f} reference {e_ref:.15f} rel {rel:.1e} (bound {bound:.1e})")
+    assert rel <= bound
+    if bound <= 1e-10:
+        # a trajectory that does not amplify rounding: the boundary tensors too
+        dC = max(np.abs(st[s]['C'][k] - final[s]['C'][k]).max() for s in st for k in range(4))
+        dE = max(np.abs(st[s]['E'][k] - final[s]['E'][k]).max() for s in st for k in range(4))
+        assert max(dC, dE) <= 1e-10

@@ -187,6 +187,66 @@ def test_als_cost_monotone_and_solver_agrees():
     assert all(c1 <= c0 + 1e-12 * abs(c0) for c0, c1 in zip(costs, costs[1:]))
 
 
+def test_gauge_fix_keeps_the_als_cost():
+    """gauge_fix (full_update.py:300-343) maps the bond problem to a frame in
+    which n12 is closer to the identity, and finalize_reduced_tensors
+    (full_update.py:123-161) maps the solution back with nzyr_inv/nzxr_inv.
+    The distance is frame-independent: for any reduced pair (b1, b2) of the
+    gauged problem, cost(b1, b2; gauged) equals cost(Yinv.b1, Xinv.b2;
+    original).  A swapped X/Y, a transposed R factor or a back-map on the
+    wrong index breaks the equality."""
+    nD, bD, pD = 6, 3, 2
+    n12, a12g = random_als_problem(nD, bD, pD, seed=21)
+    nz = ofu.positive_approx(n12.copy(), 1e-12)
+    n12 = np.einsum("xyz,XYz->xyXY", nz, nz)  # the PSD problem gauge_fix starts from
+    n12g, a12gg, xinv, yinv = ofu.gauge_fix(nz, a12g, 1e-12)
+    rng = np.random.default_rng(22)
+    for _ in range(3):
+        b1 = rng.standard_normal((nD, bD, pD))
+        b2 = rng.standard_normal((nD, bD, pD))
+        c_g = ofu.calculate_cost(b1, b2, a12gg, n12g)
+        c_o = ofu.calculate_cost(np.einsum("yz,zup->yup", yinv, b1),
+                                 np.einsum("xw,wvq->xvq", xinv, b2), a12g, n12)
+        assert c_g == pytest.approx(c_o, rel=1e-10, abs=1e-10 * abs(c_o))
+    # the solution found in the gauged frame is the original frame's optimum
+    # as well: ALS from the same start in both frames reaches the same cost
+    b1, b2, _, cost_g = ofu.als_solve(n12g, a12gg, (nD, bD, pD), niter=60, tol=1e-15)
+    c_back = ofu.calculate_cost(np.einsum("yz,zup->yup", yinv, b1),
+                                np.einsum("xw,wvq->xvq", xinv, b2), a12g, n12)
+    assert c_back == pytest.approx(cost_g, rel=1e-9)
+    _, _, _, cost_o = ofu.als_solve(n12, a12g, (nD, bD, pD), niter=400, tol=1e-15)
+    assert cost_g <= cost_o + 1e-6 * abs(cost_o)
+
+
+@pytest.mark.parametrize("gauged", [False, True])
+def test_finalize_keeps_the_bond_product(gauged):
+    """finalize_reduced_tensors (full_update.py:123-161) redistributes the
+    bond: the new pair's product over the bond index is the old product
+    P (after the optional back-map) divided by its Frobenius norm, and the
+    two halves carry equal Gram matrices diag(s/|s|).  A missing square root
+    on s, an unnormalised s or a transposed V factor breaks one of these."""
+    nD, bD, pD = 5, 3, 2
+    rng = np.random.default_rng(31 + gauged)
+    a1r = rng.standard_normal((nD, bD, pD))
+    a2r = rng.standard_normal((nD, bD, pD))
+    if gauged:
+        yinv = rng.standard_normal((nD, nD))
+        xinv = rng.standard_normal((nD, nD))
+        P = np.einsum("yup,xuq->ypxq", np.einsum("yz,zup->yup", yinv, a1r),
+                      np.einsum("xw,wvq->xvq", xinv, a2r))
+        b1, b2 = ofu.finalize_reduced_tensors(a1r, a2r, yinv, xinv)
+    else:
+        P = np.einsum("yup,xuq->ypxq", a1r, a2r)
+        b1, b2 = ofu.finalize_reduced_tensors(a1r, a2r)
+    Pn = np.einsum("yup,xuq->ypxq", b1, b2)
+    np.testing.assert_allclose(Pn, P / np.linalg.norm(P), atol=1e-13)
+    s = np.linalg.svd(P.reshape(nD * pD, nD * pD), compute_uv=False)[:bD]
+    g1 = np.einsum("yup,yvp->uv", b1, b1)
+    g2 = np.einsum("xuq,xvq->uv", b2, b2)
+    np.testing.assert_allclose(g1, np.diag(s / np.linalg.norm(s)), atol=1e-13)
+    np.testing.assert_allclose(g2, g1, atol=1e-13)
+
+
 def test_positive_approx_is_psd_square_root():
     rng = np.random.default_rng(4)
     n12 = rng.standard_normal((5, 5, 5, 5))

@@ -77,6 +77,17 @@ def _declare(lib):
                                                     c_i64, c_i64, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_int, ctypes.c_int, c_i64, c_vp, c_vp,
                                                     c_vp, c_size, c_vp]),
+        "acetn_hip_projectors_begin_graph": (ctypes.c_int, [ctypes.POINTER(AcetnSite) * 4,
+                                                            ctypes.c_int, c_i64, c_i64,
+                                                            ctypes.c_int, c_dbl, c_u64, c_u64,
+                                                            c_vp, ctypes.c_int, ctypes.c_int,
+                                                            ctypes.c_int, c_vp, c_vp, c_vp,
+                                                            c_size, c_vp]),
+        "acetn_hip_projectors_end_padded": (ctypes.c_int, [ctypes.POINTER(AcetnSite) * 4,
+                                                           ctypes.c_int, c_i64, c_i64,
+                                                           ctypes.c_int, ctypes.c_int,
+                                                           ctypes.c_int, ctypes.c_int, c_i64, c_vp,
+                                                           c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_renormalize_workspace": (c_size, [ctypes.POINTER(AcetnSite), ctypes.c_int,
                                                      c_i64, c_i64]),
         "acetn_hip_renormalize_boundary": (ctypes.c_int, [ctypes.POINTER(AcetnSite), ctypes.c_int,
@@ -471,7 +482,14 @@ class ProjectorJob:
     order; system 'half-system' | 'full-system', svd 'rsvd' | 'full-rank'."""
 
     def __init__(self, slot, sites, k, chi, oversampling, niter, cutoff, seed, stream_id,
-                 precision="fp64", system="half-system", svd="rsvd", ws_key=None):
+                 precision="fp64", system="half-system", svd="rsvd", ws_key=None, graph=None):
+        """With ``graph`` = (stream_base, counts, index) the pair is enqueued in
+        the capturable form (acetn_hip_projectors_begin_graph / _end_padded):
+        the sketch stream is stream_id + stream_base[0] (an int64 device
+        tensor), the truncation count goes to counts[index] (an int32 device
+        tensor), nothing synchronises, the workspace is private to the job,
+        and ``finish`` returns min(chi, rows) columns with those past the
+        count zero."""
         made = [make_site(*st) for st in sites]
         self.sv = [m[0] for m in made]
         self.keep = [m[1] for m in made]
@@ -486,11 +504,27 @@ class ProjectorJob:
         self.stream = slot_stream(slot, self.dev)
         nb = lib().acetn_hip_projectors_workspace(self.arr, k, chi, oversampling, self.system,
                                                   self.svd)
+        self.graph = graph
+        nspec = max(chi + oversampling, 512) if self.svd else chi + oversampling
+        if graph is not None:
+            base, counts, index = graph
+            if not (0 <= index < counts.numel()):
+                raise ValueError("ProjectorJob: count index out of range")
+            self.cols = min(chi, self.rows)
+            with torch.cuda.stream(self.stream):
+                self.ws = torch.empty(max(int(nb), 256), dtype=torch.uint8, device=self.dev)
+                self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
+                check(lib().acetn_hip_projectors_begin_graph(
+                    self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id,
+                    ptr(base), self.system, self.svd, self.prec,
+                    c_vp(counts.data_ptr() + 4 * index), ptr(self.spec), ptr(self.ws),
+                    self.ws.numel(), c_vp(self.stream.cuda_stream)))
+            self.ready = None
+            return
         # pairs queued behind each other on one slot keep their own scratch
         # until their second phase has run
         self.ws = slot_workspace(slot if ws_key is None else ("pair", ws_key), nb, self.dev)
         self.cd_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
-        nspec = max(chi + oversampling, 512) if self.svd else chi + oversampling
         with torch.cuda.stream(self.stream):
             # zero-filled on the pair's stream, ahead of the kernels that write it
             self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
@@ -503,7 +537,17 @@ class ProjectorJob:
             self.ready.record(self.stream)
 
     def finish(self):
-        """Call once ``ready`` has completed (or the stream was synchronised)."""
+        """Call once ``ready`` has completed (or the stream was synchronised);
+        in the capturable form at once."""
+        if self.graph is not None:
+            with torch.cuda.stream(self.stream):
+                P1 = torch.empty(self.rows, self.cols, dtype=torch.float64, device=self.dev)
+                P2 = torch.empty(self.rows, self.cols, dtype=torch.float64, device=self.dev)
+                check(lib().acetn_hip_projectors_end_padded(
+                    self.arr, self.k, self.chi, self.p, self.niter, self.system, self.svd,
+                    self.prec, self.cols, ptr(P1), ptr(P2), ptr(self.ws), self.ws.numel(),
+                    c_vp(self.stream.cuda_stream)))
+            return P1, P2
         n = int(self.cd_host[0])
         with torch.cuda.stream(self.stream):
             P1 = torch.empty(self.rows, n, dtype=torch.float64, device=self.dev)

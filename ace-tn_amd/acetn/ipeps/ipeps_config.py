@@ -8,6 +8,9 @@ Same keys and defaults as the reference, plus the engine's own:
                        once from torch's global generator, so
                        torch.manual_seed makes runs reproducible)
   ctmrg.conv_tol    -- optional early stop on the corner-spectrum change
+  ctmrg.graph       -- capture one iteration in a HIP graph once the boundary
+                       dimension has saturated at chi, and replay it (switched
+                       off for now: True raises NotImplementedError)
   evolution.als_start -- "plain" (SVD start, als_solver.py:117-146) or
                        "learned" (MLP proposal, judged against the plain start)
 Unknown keys raise TypeError, as the reference's dataclass(**kwargs) does.
@@ -40,6 +43,7 @@ class CTMRGConfig:
     precision: Literal["fp64", "mixed"] = "fp64"
     sketch_seed: Optional[int] = None
     conv_tol: Optional[float] = None
+    graph: bool = False
 
 
 @dataclass

@@ -56,15 +56,38 @@ def ctmrg_iteration_many(pairs):
 
 def ctmrg(ipeps, config, mover=None):
     """Runs ``config.steps`` CTMRG iterations; with ``config.conv_tol`` set,
-    stops early once the normalised corner spectra change by less than it."""
+    stops early once the normalised corner spectra change by less than it.
+
+    With ``config.graph`` set, once an iteration has run with every boundary
+    dimension at chi (so its workspaces and tile choices are those of the
+    fixed shapes), the next iteration is captured in a HIP graph and the
+    remaining ones replay it (graph.py); a replay in which a projector pair
+    kept fewer than chi singular values ends the replays, and the following
+    iterations run uncaptured until the shapes have saturated again.  The
+    capture is switched off for now (graph.CAPTURE_ENABLED), and
+    ``config.graph`` then raises NotImplementedError before any work."""
+    from . import graph as _graph
+    from .graph import CapturedIteration, saturated
+    if getattr(config, "graph", False) and not _graph.CAPTURE_ENABLED:
+        raise NotImplementedError(_graph._DISABLED)
     mover = mover if mover is not None else DirectionalMover(config, ipeps.sketch)
     disable = config.disable_progressbar
     pbar = tqdm(desc="Renormalizing boundary tensors", unit=" sweeps", disable=disable)
     prev = corner_spectra(ipeps) if config.conv_tol else None
+    use_graph = bool(getattr(config, "graph", False))
+    graph, ran_saturated = None, False
     done = 0
     for _ in range(config.steps):
         pbar.update(1)
-        ctmrg_iteration(mover, ipeps)
+        if use_graph and graph is None and ran_saturated and saturated(ipeps):
+            graph = CapturedIteration([(mover, ipeps)])
+        if graph is not None:
+            graph.replay()
+            if not graph.counts_ok():
+                graph, ran_saturated = None, False
+        else:
+            ran_saturated = use_graph and saturated(ipeps)
+            ctmrg_iteration(mover, ipeps)
         done += 1
         if config.conv_tol:
             cur = corner_spectra(ipeps)

@@ -120,7 +120,7 @@ class DirectionalMover:
                                          ws_key=ws_key)
 
 
-def run_stages(items):
+def run_stages(items, capture=None):
     """One stage of each of several independent networks, run as one round.
 
     ``items`` is a list of (mover, ipeps, moves).  Every projector pair of
@@ -136,14 +136,24 @@ def run_stages(items):
     network's Jacobi eigensolves thereby overlap the other pairs' matrix-core
     work.  The new boundary tensors are written back after everything has
     finished.  With full-system projectors the two moves of a stage depend on
-    each other, so each network then runs its stage on its own."""
+    each other, so each network then runs its stage on its own.
+
+    Under ``capture`` (a HIP graph capture of the caller's stream in
+    progress, see graph.py) the same work is enqueued without any host
+    synchronisation: the pairs take their truncation counts on the device
+    and produce chi projector columns (those past the count zero), every
+    buffer is kept alive by the capture, the workspaces are private to it,
+    and the slot streams join the caller's stream at the end instead of a
+    device synchronisation."""
     import torch
     if any(m.projector_calculator.projectors != "half-system" and len(mv) > 1
            for m, _, mv in items):
+        if capture is not None:
+            raise ValueError("graph capture: half-system projectors only")
         for m, net, mv in items:
             m.stage(net, mv)
         return
-    plans, base, joined = [], 0, set()
+    plans, base, joined = [], 0, {}
     for mover, ipeps, moves in items:
         jobs, targets = mover._plan(ipeps, moves)
         calc = mover.projector_calculator
@@ -152,12 +162,12 @@ def run_stages(items):
             # the boundary tensors the pairs read may still be in flight
             dev = ipeps[tuple(jobs[0][0][0])]['A'].device
             if str(dev) not in joined:
-                joined.add(str(dev))
+                joined[str(dev)] = dev
                 ev = torch.cuda.Event()
                 ev.record(torch.cuda.current_stream(dev))
                 for sl in range(1, _hip.N_SLOTS):
                     _hip.slot_stream(sl, dev).wait_event(ev)
-        started = calc.start_many(ipeps, jobs, base=base)
+        started = calc.start_many(ipeps, jobs, base=base, capture=capture)
         plans.append({"mover": mover, "ipeps": ipeps, "targets": targets, "started": started,
                       "base": base, "proj": [None] * len(jobs), "done": [None] * len(jobs),
                       "results": [None] * len(targets)})
@@ -186,15 +196,25 @@ def run_stages(items):
                         st = ipeps[s1]
                         parts[t] = (_hip.RenormalizeParts(
                             (st['A'], st['C'], st['E']), k, ipeps.dims['chi'],
-                            first_pair="j" if a == j else "i", slot=slot,
-                            precision=calc.precision, ws_key=g), a)
+                            first_pair="j" if a == j else "i",
+                            slot=None if capture is not None else slot,
+                            precision=calc.precision,
+                            ws_key=None if capture is not None else g), a)
                         parts[t][0].first(flat(proj[a][0]), flat(proj[a][1]))
                 if parts[t] is not None and proj[i] is not None and proj[j] is not None:
                     b = i if parts[t][1] == j else j
                     stream.wait_event(done[b])
                     with torch.cuda.stream(stream):
                         results[t] = parts[t][0].second(flat(proj[b][0]), flat(proj[b][1]))
-    torch.cuda.synchronize()
+        pl["parts"] = parts
+    if capture is None:
+        torch.cuda.synchronize()
+    else:
+        for dev in joined.values():
+            cur = torch.cuda.current_stream(dev)
+            for sl in range(1, _hip.N_SLOTS):
+                cur.wait_stream(_hip.slot_stream(sl, dev))
+        capture.keep.append(plans)
     for pl in plans:
         for (s1, s2, i, j, k), (Ca, Cb, Eo) in zip(pl["targets"], pl["results"]):
             dst = pl["ipeps"][s2]

@@ -70,12 +70,14 @@ class ProjectorCalculator:
         torch.cuda.synchronize()
         return [pr for _, pr, _ in out]
 
-    def start_many(self, ipeps, jobs, system=None, base=0):
+    def start_many(self, ipeps, jobs, system=None, base=0, capture=None):
         """Enqueue every projector pair up to its truncation count; returns
         the started jobs in ``jobs`` order.  Pair i is round-wide pair
         base + i (``base`` counts the pairs of networks started before it in
         the same round) and runs on slot (base + i) % N_SLOTS with a
-        workspace of its own."""
+        workspace of its own.  Under ``capture`` (a graph capture in
+        progress) the pairs are enqueued in the capturable form, their
+        sketch streams numbered by the capture."""
         system = system or ("full-system" if self.calculate == self.calculate_full_system
                             else "half-system")
         if self.svd_type not in ("rsvd", "full-rank"):
@@ -88,12 +90,16 @@ class ProjectorCalculator:
             sites = [tuple(s) for s in sites]
             for s in sites:
                 ipeps[s]  # raises ValueError for an unknown site
-            stream_id = self.sketch.take() if self.svd_type == "rsvd" else 0
+            graph = None
+            if capture is not None:
+                stream_id, graph = capture.next_pair(self)
+            else:
+                stream_id = self.sketch.take() if self.svd_type == "rsvd" else 0
             started.append(_hip.ProjectorJob(slot, [self._site(ipeps, s) for s in sites], k % 4,
                                              chi, self.rsvd_oversampling, self.rsvd_niter,
                                              self.svd_cutoff, self.sketch.seed, stream_id,
                                              precision=self.precision, system=system,
-                                             svd=self.svd_type, ws_key=ws_key))
+                                             svd=self.svd_type, ws_key=ws_key, graph=graph))
         return started
 
     def finish_each(self, ipeps, started):
@@ -104,9 +110,13 @@ class ProjectorCalculator:
         import torch
         D = ipeps.dims['bond']
         for idx, job in enumerate(started):
-            job.ready.synchronize()
+            if job.ready is not None:
+                job.ready.synchronize()
             P1, P2 = job.finish()
-            self.recent_spectra.append((job.spec, P1.shape[1]))
+            # the kept count: a device scalar for a captured pair (written on
+            # every replay)
+            cd = P1.shape[1] if job.graph is None else job.graph[1][job.graph[2]]
+            self.recent_spectra.append((job.spec, cd))
             done = torch.cuda.Event()
             done.record(job.stream)
             rows = P1.shape[0]
@@ -124,6 +134,7 @@ class ProjectorCalculator:
         import torch
         out = []
         for spec, cd in list(self.recent_spectra)[-count:]:
+            cd = int(cd)
             s2 = spec.double().cpu() ** 2
             tot = float(s2.sum())
             out.append(float(s2[cd:].sum()) / tot if tot > 0 else 0.0)

@@ -162,9 +162,15 @@ struct HalfOut {
   double* P1;
   double* P2;
   int64_t* host_cD;   // phase 0: written after a stream sync
-  int* pinned_cD;     // phase 1: async copy target (pinned host memory)
-  int cD_in;          // phase 2: cD_new read by the host after phase 1
+  int* pinned_cD;     // phase 1: async copy target (pinned host memory), or null
+  int cD_in;          // phase 2: cD_new read by the host after phase 1 (padded: min(chi, q))
   double* spectrum;
+  // graph-capturable form: the sketch's Philox stream is sstream +
+  // stream_base[0] (device); phase 1 also copies cD_new to dev_cD (device);
+  // phase 2 writes cD_in columns, those at and past the device count zero
+  const uint64_t* stream_base = nullptr;
+  int32_t* dev_cD = nullptr;
+  bool padded = false;
 };
 
 // A product with an enlarged corner as its A operand: from the corner's bf16
@@ -245,7 +251,7 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   } else {
     q1n.p = q1t.p = q4n.p = q4t.p = nullptr;
   }
-  if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st);
+  if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st, out.stream_base);
   // mixed precision: the sketch and the power iterations only steer the range
   // finder (bf16x3); the products after the final orthonormalisation carry the
   // basis itself (bf16x6)
@@ -284,7 +290,10 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
     count_kept(w, (int)q, (int)chi, cutoff, s, cD_dev, c.st);
     if (out.spectrum) ACETN_HIP_CHECK(hipMemcpyAsync(out.spectrum, s, sizeof(double) * q, hipMemcpyDeviceToDevice, c.st));
     if (phase == 1) {
-      ACETN_HIP_CHECK(hipMemcpyAsync(out.pinned_cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+      if (out.pinned_cD)
+        ACETN_HIP_CHECK(hipMemcpyAsync(out.pinned_cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+      if (out.dev_cD)
+        ACETN_HIP_CHECK(hipMemcpyAsync(out.dev_cD, cD_dev, sizeof(int), hipMemcpyDeviceToDevice, c.st));
       return;
     }
     ACETN_HIP_CHECK(hipMemcpyAsync(&cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
@@ -292,12 +301,15 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
     *out.host_cD = cD;
   }
   ACETN_REQUIRE(cD >= 1 && cD <= std::min<int64_t>(chi, q), "half system: no singular value above the cutoff");
+  ACETN_REQUIRE(!out.padded || cD == std::min<int64_t>(chi, q),
+                "projectors_end_padded: the column count must be min(chi, q)");
   // P1 = Q1^H U S_n^-1/2 = Wq (UB S_n^-1/2)
   c.split = 0;
-  scale_columns(UB, (int)q, s, cD, 0, X, c.st);
+  const int* kept = out.padded ? cD_dev : nullptr;
+  scale_columns(UB, (int)q, s, cD, 0, X, c.st, kept);
   mm(c, (int)kk, cD, (int)q, false, Wq, q, false, X, cD, out.P1, cD);
   // P2 = Q4 V S_n^-1/2,  V = B^T UB S^-1
-  scale_columns(UB, (int)q, s, cD, 1, X, c.st);
+  scale_columns(UB, (int)q, s, cD, 1, X, c.st, kept);
   mm(c, (int)n, cD, (int)q, false, BT, q, false, X, cD, Vp, cD);
   c.split = c.mixed ? 3 : 0;
   mm_corner(c, q4n, (int)kk, cD, (int)n, false, Q4, n, Vp, cD, out.P2, cD);
@@ -410,7 +422,7 @@ void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, 
   const int qi = (int)q;
   SplitScope sc(c, 2);
   if (svd == kSvdRsvd) {
-    if (c.launch) gaussian_sketch(Om, f, q, seed, sstream, c.st);
+    if (c.launch) gaussian_sketch(Om, f, q, seed, sstream, c.st, out.stream_base);
     apply_R(Om, qi, W);
     apply_L(W, qi, Y);
     for (int it = 0; it < niter; ++it) {
@@ -458,7 +470,10 @@ void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, 
     count_kept(w, qi, (int)chi, cutoff, sv, cD_dev, c.st);
     if (out.spectrum) ACETN_HIP_CHECK(hipMemcpyAsync(out.spectrum, sv, sizeof(double) * q, hipMemcpyDeviceToDevice, c.st));
     if (phase == 1) {
-      ACETN_HIP_CHECK(hipMemcpyAsync(out.pinned_cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+      if (out.pinned_cD)
+        ACETN_HIP_CHECK(hipMemcpyAsync(out.pinned_cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
+      if (out.dev_cD)
+        ACETN_HIP_CHECK(hipMemcpyAsync(out.dev_cD, cD_dev, sizeof(int), hipMemcpyDeviceToDevice, c.st));
       return;
     }
     ACETN_HIP_CHECK(hipMemcpyAsync(&cD, cD_dev, sizeof(int), hipMemcpyDeviceToHost, c.st));
@@ -466,9 +481,12 @@ void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, 
     *out.host_cD = cD;
   }
   ACETN_REQUIRE(cD >= 1 && cD <= std::min<int64_t>(chi, q), "projectors: no singular value above the cutoff");
+  ACETN_REQUIRE(!out.padded || cD == std::min<int64_t>(chi, q),
+                "projectors_end_padded: the column count must be min(chi, q)");
   c.split = 0;
   // P1 = L^T U S^-1/2 (U = Qm UB in the rSVD branch, UB in the full-rank one)
-  scale_columns(UB, qi, sv, cD, 0, X, c.st);
+  const int* kept = out.padded ? cD_dev : nullptr;
+  scale_columns(UB, qi, sv, cD, 0, X, c.st, kept);
   if (svd == kSvdRsvd) {
     mm(c, (int)cc, cD, qi, false, W, q, false, X, cD, out.P1, cD);
   } else {
@@ -477,7 +495,7 @@ void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, 
     c.split = 0;
   }
   // P2 = R V S^-1/2, V = B^T UB S^-1
-  scale_columns(UB, qi, sv, cD, 1, X, c.st);
+  scale_columns(UB, qi, sv, cD, 1, X, c.st, kept);
   mm(c, (int)f, cD, qi, false, BT, q, false, X, cD, Vp, cD);
   c.split = c.mixed ? 3 : 0;
   apply_R(Vp, cD, out.P2);
@@ -1023,6 +1041,53 @@ int acetn_hip_projectors_end(const acetn_site* const sites[4], int k, int64_t ch
     check_precision(precision);
     ACETN_REQUIRE(P1 && P2, "projectors_end: null output");
     HalfOut o{P1, P2, nullptr, nullptr, (int)cD_new, nullptr};
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
+      if (system == kSysHalf && svd_type == kSvdRsvd)
+        half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
+      else
+        general_projectors(c, sites, k, system, svd_type, chi, oversampling, niter, 0.0, 0, 0, o,
+                           2);
+    });
+  });
+}
+
+int acetn_hip_projectors_begin_graph(const acetn_site* const sites[4], int k, int64_t chi,
+                                     int64_t oversampling, int niter, double svd_cutoff,
+                                     uint64_t seed, uint64_t sketch_stream,
+                                     const uint64_t* dev_stream_base, int system, int svd_type,
+                                     int precision, int32_t* dev_cD_new, double* spectrum,
+                                     void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_projector_args(sites, system, svd_type);
+    check_precision(precision);
+    ACETN_REQUIRE(dev_stream_base && dev_cD_new, "projectors_begin_graph: null device pointer");
+    ACETN_REQUIRE(niter >= 0 && niter <= 16, "projectors: niter out of range");
+    HalfOut o{nullptr, nullptr, nullptr, nullptr, 0, spectrum};
+    o.stream_base = dev_stream_base;
+    o.dev_cD = dev_cD_new;
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      c.mixed = precision == ACETN_HIP_MIXED;
+      if (system == kSysHalf && svd_type == kSvdRsvd)
+        half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, svd_cutoff, seed,
+                        sketch_stream, o, 1);
+      else
+        general_projectors(c, sites, k, system, svd_type, chi, oversampling, niter, svd_cutoff,
+                           seed, sketch_stream, o, 1);
+    });
+  });
+}
+
+int acetn_hip_projectors_end_padded(const acetn_site* const sites[4], int k, int64_t chi,
+                                    int64_t oversampling, int niter, int system, int svd_type,
+                                    int precision, int64_t cD_cols, double* P1, double* P2,
+                                    void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_projector_args(sites, system, svd_type);
+    check_precision(precision);
+    ACETN_REQUIRE(P1 && P2, "projectors_end_padded: null output");
+    HalfOut o{P1, P2, nullptr, nullptr, (int)cD_cols, nullptr};
+    o.padded = true;
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       c.mixed = precision == ACETN_HIP_MIXED;
       if (system == kSysHalf && svd_type == kSvdRsvd)

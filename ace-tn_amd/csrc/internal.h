@@ -215,9 +215,10 @@ void double_layer_f32(const DLDesc& g, const double* X, const double* A, double*
 // ---------------------------------------------------------------------------
 // small dense linear algebra (one workgroup per problem)
 // ---------------------------------------------------------------------------
-// Gaussian sketch (Philox4x32-10 + Box-Muller), n x q row-major.
+// Gaussian sketch (Philox4x32-10 + Box-Muller), n x q row-major, Philox
+// stream ``stream`` (+ stream_base[0], read on the device, when given).
 void gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed, uint64_t stream,
-                     hipStream_t s);
+                     hipStream_t s, const uint64_t* stream_base = nullptr);
 
 // Cholesky-based orthonormalisation helper: given the q x q Gram matrix G
 // (row-major, ld=q), write the upper-triangular inverse factor Rinv (q x q,
@@ -258,8 +259,9 @@ size_t jacobi_svd_scratch_elems(int rows, int cols);
 // X[i, z] = Ub[i, z] * f(s[z]) for z < cD, with f selected by ``mode``:
 //   mode 0: (s[z]/s0)^-1/2          (P1 factor)
 //   mode 1: 1 / (s[z] * sqrt(s[z]/s0)) (P2 factor through V = B^T U S^-1)
+// With ``kept`` (a device count), the columns z >= kept[0] are written as zero.
 void scale_columns(const double* Ub, int q, const double* s, int cD, int mode, double* X,
-                   hipStream_t st);
+                   hipStream_t st, const int* kept = nullptr);
 // cD_new = min(chi, #{s/s0 > cutoff}) with s = sqrt(max(w, 0)).
 void count_kept(const double* w, int q, int chi, double cutoff, double* s_out, int* cD_out,
                 hipStream_t st);

@@ -40,7 +40,12 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
   }
 }
 
-__global__ void sketch_kernel(double* out, int64_t total, uint64_t seed, uint64_t stream) {
+__global__ void sketch_kernel(double* out, int64_t total, uint64_t seed, uint64_t stream0,
+                              const uint64_t* __restrict__ stream_base) {
+  // the stream number: a constant, plus a device counter when one is given
+  // (read when the kernel runs, so a captured launch draws a new sketch on
+  // every replay once the counter has moved)
+  const uint64_t stream = stream0 + (stream_base ? stream_base[0] : 0);
   const int64_t pairs = (total + 1) / 2;
   for (int64_t call = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; call < pairs;
        call += (int64_t)gridDim.x * blockDim.x) {
@@ -1030,10 +1035,16 @@ __global__ void count_kept_kernel(const double* __restrict__ w, int q, int chi, 
 
 __global__ void scale_columns_kernel(const double* __restrict__ Ub, int q,
                                      const double* __restrict__ s, int cD, int mode,
-                                     double* __restrict__ X) {
+                                     const int* __restrict__ kept, double* __restrict__ X) {
   const double s0 = s[0];
+  // with ``kept`` (a count on the device), columns z >= kept[0] are zero
+  const int nk = kept ? kept[0] : cD;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < q * cD; e += gridDim.x * blockDim.x) {
     const int i = e / cD, z = e % cD;
+    if (z >= nk) {
+      X[e] = 0.0;
+      continue;
+    }
     const double sz = s[z];
     const double sn = sz / s0;
     const double f = (mode == 0) ? 1.0 / sqrt(sn) : 1.0 / (sz * sqrt(sn));
@@ -1110,12 +1121,13 @@ size_t packed_bytes(int n) { return sizeof(double) * (size_t)n * (n + 1) / 2; }
 }  // namespace
 
 void gaussian_sketch(double* out, int64_t n, int64_t q, uint64_t seed, uint64_t stream,
-                     hipStream_t s) {
+                     hipStream_t s, const uint64_t* stream_base) {
   const int64_t total = n * q;
   if (total <= 0) return;
   const int64_t pairs = (total + 1) / 2;
   const int blocks = (int)std::min<int64_t>((pairs + 255) / 256, 4096);
-  hipLaunchKernelGGL(sketch_kernel, dim3(blocks), dim3(256), 0, s, out, total, seed, stream);
+  hipLaunchKernelGGL(sketch_kernel, dim3(blocks), dim3(256), 0, s, out, total, seed, stream,
+                     stream_base);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 
@@ -1232,10 +1244,11 @@ void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_
 }
 
 void scale_columns(const double* Ub, int q, const double* s, int cD, int mode, double* X,
-                   hipStream_t st) {
+                   hipStream_t st, const int* kept) {
   if (cD <= 0) return;
   const int blocks = std::min((q * cD + 255) / 256, 1024);
-  hipLaunchKernelGGL(scale_columns_kernel, dim3(blocks), dim3(256), 0, st, Ub, q, s, cD, mode, X);
+  hipLaunchKernelGGL(scale_columns_kernel, dim3(blocks), dim3(256), 0, st, Ub, q, s, cD, mode,
+                     kept, X);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

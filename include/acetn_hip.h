@@ -176,6 +176,30 @@ int acetn_hip_projectors_end(const acetn_site* const sites[4], int k, int64_t ch
                              int precision, int64_t cD_new, double* P1, double* P2,
                              void* workspace, size_t ws_bytes, void* stream);
 
+/* The same two phases in a form that HIP stream capture can record (no
+ * counterpart in the reference, whose loop reads cD_new on the host,
+ * projectors.py:163-164; for callers that capture a whole CTMRG iteration
+ * once the boundary dimension has saturated at chi):
+ *   _begin_graph draws the sketch from Philox stream sketch_stream +
+ *   dev_stream_base[0], read on the device when the sketch kernel runs (a
+ *   caller that advances that counter between replays draws fresh sketches),
+ *   and copies cD_new to dev_cD_new (device memory) instead of host memory;
+ *   _end_padded writes cD_cols = min(chi, q) projector columns whatever
+ *   cD_new is (q = chi + oversampling, capped by the matrix sizes): the
+ *   columns at and past cD_new are zero, an exact embedding of the reference's
+ *   truncated projectors in fixed shapes.  Neither call synchronises.
+ * Both must receive the same sites, dimensions and workspace. */
+int acetn_hip_projectors_begin_graph(const acetn_site* const sites[4], int k, int64_t chi,
+                                     int64_t oversampling, int niter, double svd_cutoff,
+                                     uint64_t seed, uint64_t sketch_stream,
+                                     const uint64_t* dev_stream_base, int system, int svd_type,
+                                     int precision, int32_t* dev_cD_new, double* spectrum,
+                                     void* workspace, size_t ws_bytes, void* stream);
+int acetn_hip_projectors_end_padded(const acetn_site* const sites[4], int k, int64_t chi,
+                                    int64_t oversampling, int niter, int system, int svd_type,
+                                    int precision, int64_t cD_cols, double* P1, double* P2,
+                                    void* workspace, size_t ws_bytes, void* stream);
+
 /* Boundary renormalisation of one (s1 -> s2) target.
  * Replaces DirectionalMover.renormalize_boundary / renormalize_cj1 /
  * renormalize_cj2 / renormalize_ej, acetn/renormalization/directional_mover.py:273-366.

@@ -50,3 +50,14 @@ def test_config_validation():
         IpepsConfig(TN={'nx': 2, 'ny': 2}, ctmrg={'no_such_key': 1})
     with pytest.raises(NotImplementedError):
         IpepsConfig(TN={'nx': 2, 'ny': 2}, dtype="float32")
+
+
+def test_ctmrg_graph_config_raises_while_switched_off():
+    """config.graph=True fails before any work while the capture is off
+    (runs without a GPU: the check precedes every device call)."""
+    from acetn.renormalization.ctmrg import ctmrg
+
+    class C:
+        graph = True
+    with pytest.raises(NotImplementedError, match="switched off"):
+        ctmrg(None, C())

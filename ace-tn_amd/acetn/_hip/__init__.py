@@ -136,6 +136,11 @@ def _declare(lib):
         "acetn_hip_als_workspace": (c_size, [c_i64, c_i64]),
         "acetn_hip_als_initialize": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_size,
                                                     c_vp]),
+        "acetn_hip_als_batch_workspace": (c_size, [c_i64, c_i64, c_i64]),
+        "acetn_hip_als_solve_batch": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64,
+                                                     ctypes.c_int, c_dbl, ctypes.c_int, c_dbl,
+                                                     ctypes.c_int, c_vp, c_vp, c_vp, c_size,
+                                                     c_vp]),
         "acetn_hip_als_solve": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, ctypes.c_int,
                                                c_dbl, ctypes.c_int, c_dbl, ctypes.POINTER(c_i64),
                                                ctypes.POINTER(c_dbl),
@@ -780,6 +785,37 @@ def als_solve(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
                                     float(epsilon), ctypes.byref(sweeps), ctypes.byref(cost),
                                     ctypes.byref(info), ptr(ws), ws.numel(), stream_handle()))
     return b1, b2, sweeps.value, cost.value, info.value
+
+
+def als_solve_batch(n12, a12g, D, niter, tol, method, epsilon, starts=None):
+    """``count`` ALS problems in one launch (acetn_hip_als_solve_batch):
+    n12 (count, nD, nD, nD, nD), a12g (count, nD, nD, d, d) on the device;
+    ``starts`` = (a1r, a2r), each (count, nD, D, d), or None for the plain
+    start.  Returns (a1r, a2r, sweeps, cost, flags) as device tensors; no
+    synchronisation."""
+    if method not in ALS_METHODS:
+        raise ValueError(f"Unknown ALS method: {method}")
+    count, nD = int(a12g.shape[0]), int(a12g.shape[1])
+    d = int(a12g.shape[3])
+    if tuple(n12.shape) != (count, nD, nD, nD, nD) or tuple(a12g.shape) != (count, nD, nD, d, d):
+        raise ValueError("als_solve_batch: n12 / a12g shapes do not match")
+    dev = a12g.device
+    n12, a12g = n12.contiguous(), a12g.contiguous()
+    if starts is None:
+        a1 = torch.empty(count, nD, D, d, dtype=torch.float64, device=dev)
+        a2 = torch.empty_like(a1)
+    else:
+        a1, a2 = (t.clone().contiguous() for t in starts)
+        if tuple(a1.shape) != (count, nD, D, d) or tuple(a2.shape) != (count, nD, D, d):
+            raise ValueError("als_solve_batch: start shapes do not match")
+    cost = torch.empty(count, dtype=torch.float64, device=dev)
+    info = torch.empty(count, 2, dtype=torch.int32, device=dev)
+    ws = workspace(lib().acetn_hip_als_batch_workspace(count, D, d), dev)
+    check(lib().acetn_hip_als_solve_batch(ptr(n12), ptr(a12g), ptr(a1), ptr(a2), count, D, d,
+                                          int(niter), float(tol), ALS_METHODS[method],
+                                          float(epsilon), 1 if starts is None else 0, ptr(cost),
+                                          ptr(info), ptr(ws), ws.numel(), stream_handle()))
+    return a1, a2, info[:, 0], cost, info[:, 1]
 
 
 def als_solve_noalloc(n12, a12g, a1r, a2r, niter, tol, method, epsilon):

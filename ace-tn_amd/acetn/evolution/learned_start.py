@@ -4,20 +4,23 @@ A 3-layer perceptron (hidden width 512 by default, bf16 matrix-core
 products with FP32 accumulation, hand-written forward / backward / Adam
 kernels in libacetn_hip) maps one ALS problem of the full update -- the
 gauge-fixed norm tensor n12[y][x][Y][X] and the gated bond tensor
-a12g[y][x][p][q] -- to a correction of the bond product:
+a12g[y][x][p][q] -- to a correction of the plain start's bond product
+a12p = a1r a2r (the truncated SVD of a12g, als_solver.py:117-146):
 
-    a12_proposed = a12g + ||a12g|| * MLP(features)
+    a12_proposed = a12p + ||a12g|| * MLP(features)
 
-where the training target is (a12* - a12g) / ||a12g|| with a12* = a1r* a2r*
-the converged ALS solution (a bond-gauge invariant, so the target is well
-defined).  The proposal is factored like the plain start (truncated SVD,
-acetn_hip_als_initialize); it replaces the plain start only when its cost
-does not exceed the plain start's (both evaluated with acetn_hip_als_solve at
-zero sweeps), so the plain ALS stays the fallback and the judge.
+where the training target is (a12* - a12p) / ||a12g|| with a12* = a1r* a2r*
+the converged ALS solution (bond products are bond-gauge invariant, so the
+target is well defined; the network only has to learn how the norm tensor's
+metric moves the optimum away from the Frobenius truncation).  The proposal
+is factored like the plain start (truncated SVD, acetn_hip_als_initialize);
+it replaces the plain start only when its cost does not exceed the plain
+start's (both evaluated with acetn_hip_als_solve at zero sweeps), so the
+plain ALS stays the fallback and the judge.
 
-Features (FP32, one row per problem): a12g/||a12g||, n12g/||n12g|| with
-n12g = sum_{y,x} n12[y][x][Y][X] a12g[y][x][p][q] (the ALS right-hand side),
-and n12/||n12||.
+Features (FP32, one row per problem): a12g/||a12g||, a12p/||a12g||,
+n12g/||n12g|| with n12g = sum_{y,x} n12[y][x][Y][X] a12g[y][x][p][q] (the
+ALS right-hand side), and n12/||n12||.
 """
 import math
 
@@ -28,17 +31,26 @@ from .. import _hip
 
 def feature_sizes(nD, d):
     nE = nD * nD * d * d
-    return 2 * nE + nD ** 4, nE
+    return 3 * nE + nD ** 4, nE
 
 
-def features(n12, a12g):
-    """Feature row (FP32, device) of one ALS problem."""
+def features(n12, a12g, a12p):
+    """Feature row (FP32, device) of one ALS problem; a12p is the plain
+    start's bond product."""
     nD, _, d, _ = a12g.shape
     a = a12g.reshape(nD * nD, d * d)
     n = n12.reshape(nD * nD, nD * nD)
     g = _hip.gemm(n, a, ta=True)                      # n12g[(Y,X)][(p,q)]
-    parts = [a / a.norm(), g / g.norm(), n / n.norm()]
+    an = a.norm()
+    parts = [a / an, a12p / an, g / g.norm(), n / n.norm()]
     return torch.cat([p.reshape(-1) for p in parts]).to(torch.float32)
+
+
+def plain_product(a12g):
+    """The plain start's bond product a12p (device)."""
+    nD, _, d, _ = a12g.shape
+    p1, p2 = _hip.als_initialize(a12g, nD // d)
+    return bond_product(p1, p2)
 
 
 def bond_product(a1r, a2r):
@@ -50,8 +62,8 @@ def bond_product(a1r, a2r):
     return P.reshape(nD, d, nD, d).permute(0, 2, 1, 3).contiguous()
 
 
-def target(a12g, a1r, a2r):
-    return ((bond_product(a1r, a2r) - a12g) / a12g.norm()).reshape(-1).to(torch.float32)
+def target(a12g, a12p, a1r, a2r):
+    return ((bond_product(a1r, a2r) - a12p) / a12g.norm()).reshape(-1).to(torch.float32)
 
 
 class MLP:
@@ -77,6 +89,22 @@ class MLP:
 
     def forward(self, X):
         return _hip.mlp_forward(self.params, self.fin, self.hidden, self.fout, X)
+
+    def save(self, path):
+        """Plain tensors only (loadable with torch.load(weights_only=True))."""
+        torch.save({"format": "acetn-als-start-mlp/1", "fin": self.fin, "hidden": self.hidden,
+                    "fout": self.fout, "params": self.params.detach().cpu()}, path)
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        st = torch.load(path, weights_only=True, map_location="cpu")
+        if st.get("format") != "acetn-als-start-mlp/1":
+            raise ValueError(f"{path}: not a saved ALS-start network")
+        mlp = cls(int(st["fin"]), int(st["fout"]), hidden=int(st["hidden"]), device=device)
+        if mlp.params.numel() != st["params"].numel():
+            raise ValueError(f"{path}: parameter count does not match the layer sizes")
+        mlp.params.copy_(st["params"].to(device=device, dtype=torch.float32))
+        return mlp
 
     def train_step(self, X, T, lr, weight_decay=0.0):
         loss, _ = _hip.mlp_backward(self.params, self.fin, self.hidden, self.fout, X, T, self.grad)
@@ -118,13 +146,15 @@ class MLPProposer:
         self.used = 0
         self.rejected = 0
 
-    def propose(self, n12, a12g):
-        y = self.mlp.forward(features(n12, a12g).unsqueeze(0))[0]
-        return a12g + a12g.norm() * y.to(torch.float64).reshape(a12g.shape)
+    def propose(self, n12, a12g, a12p):
+        y = self.mlp.forward(features(n12, a12g, a12p).unsqueeze(0))[0]
+        return a12p + a12g.norm() * y.to(torch.float64).reshape(a12g.shape)
 
     def choose(self, n12, a12g, a1r, a2r):
+        """(a1r, a2r) is the plain start."""
         bD = a1r.shape[1]
-        p1, p2 = _hip.als_initialize(self.propose(n12, a12g).contiguous(), bD)
+        a12p = bond_product(a1r, a2r)
+        p1, p2 = _hip.als_initialize(self.propose(n12, a12g, a12p).contiguous(), bD)
         if start_cost(n12, a12g, p1, p2, self.config) <= start_cost(n12, a12g, a1r, a2r,
                                                                     self.config):
             self.used += 1
@@ -161,9 +191,12 @@ def evaluate(proposer, problems, config, tol=1e-10):
 
 def dataset(problems):
     """Stack (features, targets) of recorded ALS problems."""
-    X = torch.stack([features(p['n12'], p['a12g']) for p in problems])
-    T = torch.stack([target(p['a12g'], p['a1r'], p['a2r']) for p in problems])
-    return X, T
+    X, T = [], []
+    for p in problems:
+        a12p = plain_product(p['a12g'])
+        X.append(features(p['n12'], p['a12g'], a12p))
+        T.append(target(p['a12g'], a12p, p['a1r'], p['a2r']))
+    return torch.stack(X), torch.stack(T)
 
 
 def generate_problems(ipeps, model, dtau, steps, config):

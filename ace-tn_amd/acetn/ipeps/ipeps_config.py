@@ -13,6 +13,7 @@ Same keys and defaults as the reference, plus the engine's own:
                        off for now: True raises NotImplementedError)
   evolution.als_start -- "plain" (SVD start, als_solver.py:117-146) or
                        "learned" (MLP proposal, judged against the plain start)
+  evolution.als_start_model -- path of the saved network for "learned"
 Unknown keys raise TypeError, as the reference's dataclass(**kwargs) does.
 """
 import os
@@ -59,6 +60,7 @@ class EvolutionConfig:
     als_epsilon: float = 1e-12
     disable_progressbar: bool = False
     als_start: Literal["plain", "learned"] = "plain"
+    als_start_model: Optional[str] = None
 
 
 @dataclass

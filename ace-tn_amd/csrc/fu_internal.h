@@ -23,6 +23,9 @@ void householder_qr(const double* A, int m, int n, double* W, double* Q, double*
 // One-sided Jacobi SVD, A (m x n, m <= 64, n <= 32) = U diag(s) V^T,
 // U (m x n), V (n x n), s descending.
 void svd_small(const double* A, int m, int n, double* U, double* sv, double* V, hipStream_t s);
+// ``count`` problems, matrices (A, U, V) ``stride`` doubles apart, sv n apart.
+void svd_small_batch(const double* A, int m, int n, double* U, double* sv, double* V, int count,
+                     int64_t stride, hipStream_t s);
 
 void lower_to_full(double* M, int n, hipStream_t s);
 void add_diag(double* M, int n, double shift, hipStream_t s);
@@ -48,8 +51,11 @@ void su_split(const double* U, const double* S, const double* V, int nD, int bD,
 // (1: a Cholesky factorisation failed and LU was used; 2: singular system).
 size_t als_work_elems(int nD, int D, int d);
 size_t als_lds_bytes(int nD, int D, int d);
+// With count > 1, problem b's n12, a12g, a1r, a2r, gwork, out_cost and
+// out_info follow problem b-1's contiguously (one workgroup per problem).
 void als_solve_device(const double* n12, const double* a12g, double* a1r, double* a2r, int nD,
                       int D, int d, int niter, double tol, double eps, double* gwork,
-                      double* out_cost, int* out_info, hipStream_t s, int method = 0);
+                      double* out_cost, int* out_info, hipStream_t s, int method = 0,
+                      int count = 1);
 
 }  // namespace acetn

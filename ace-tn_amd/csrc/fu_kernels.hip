@@ -161,7 +161,14 @@ constexpr int SVD_NT = 1024;
 __global__ __launch_bounds__(SVD_NT) void svd_small_kernel(const double* __restrict__ A, int m,
                                                            int n, double* __restrict__ U,
                                                            double* __restrict__ s,
-                                                           double* __restrict__ V) {
+                                                           double* __restrict__ V,
+                                                           int64_t batch_stride) {
+  // workgroup b decomposes problem b: matrices batch_stride doubles apart,
+  // singular values n apart
+  A += blockIdx.x * batch_stride;
+  U += blockIdx.x * batch_stride;
+  V += blockIdx.x * batch_stride;
+  s += (int64_t)blockIdx.x * n;
   __shared__ double a[kSvdMaxN + 1][64];   // columns, padded to even count
   __shared__ double v[kSvdMaxN + 1][kSvdMaxN + 1];
   __shared__ double nrm[kSvdMaxN + 1];
@@ -729,7 +736,22 @@ __global__ __launch_bounds__(ALS_NT) void als_kernel(const double* __restrict__ 
                                                      int d, int niter, double tol, double eps,
                                                      int method, double* __restrict__ gwork,
                                                      double* __restrict__ out_cost,
-                                                     int* __restrict__ out_info) {
+                                                     int* __restrict__ out_info,
+                                                     int64_t n12_stride, int64_t gwork_stride) {
+  // workgroup b solves problem b of a batch: n12 n12_stride doubles apart,
+  // a12g nE apart, a1r/a2r nR*d apart, scratch gwork_stride apart, one cost
+  // and two info words each
+  {
+    const int64_t b = blockIdx.x;
+    const int64_t nE_ = (int64_t)nD * nD * d * d, nRd = (int64_t)nD * D * d;
+    n12 += b * n12_stride;
+    a12g += b * nE_;
+    a1r += b * nRd;
+    a2r += b * nRd;
+    gwork += b * gwork_stride;
+    out_cost += b;
+    out_info += 2 * b;
+  }
   extern __shared__ double lds[];
   __shared__ int flag;
   __shared__ double s_d1, s_d2;
@@ -814,7 +836,17 @@ void householder_qr(const double* A, int m, int n, double* W, double* Q, double*
 
 void svd_small(const double* A, int m, int n, double* U, double* sv, double* V, hipStream_t s) {
   ACETN_REQUIRE(m >= 1 && m <= 64 && n >= 1 && n <= kSvdMaxN, "svd_small: unsupported shape");
-  hipLaunchKernelGGL(svd_small_kernel, dim3(1), dim3(SVD_NT), 0, s, A, m, n, U, sv, V);
+  hipLaunchKernelGGL(svd_small_kernel, dim3(1), dim3(SVD_NT), 0, s, A, m, n, U, sv, V,
+                     (int64_t)0);
+  ACETN_HIP_CHECK(hipGetLastError());
+}
+
+void svd_small_batch(const double* A, int m, int n, double* U, double* sv, double* V, int count,
+                     int64_t stride, hipStream_t s) {
+  ACETN_REQUIRE(m >= 1 && m <= 64 && n >= 1 && n <= kSvdMaxN, "svd_small: unsupported shape");
+  ACETN_REQUIRE(count >= 1 && stride >= (int64_t)m * n, "svd_small_batch: bad batch layout");
+  hipLaunchKernelGGL(svd_small_kernel, dim3(count), dim3(SVD_NT), 0, s, A, m, n, U, sv, V,
+                     stride);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 
@@ -882,8 +914,9 @@ size_t als_work_elems(int nD, int D, int d) {
 
 void als_solve_device(const double* n12, const double* a12g, double* a1r, double* a2r, int nD,
                       int D, int d, int niter, double tol, double eps, double* gwork,
-                      double* out_cost, int* out_info, hipStream_t s, int method) {
+                      double* out_cost, int* out_info, hipStream_t s, int method, int count) {
   ACETN_REQUIRE(method == 0 || method == 1, "als: unknown method");
+  ACETN_REQUIRE(count >= 1, "als: empty batch");
   ACETN_REQUIRE(method == 0 || nD * D <= 256, "als: pinv needs nD * D <= 256");
   const size_t lds = als_lds_bytes(nD, D, d);
   ACETN_REQUIRE(lds <= 150 * 1024, "als: reduced tensors too large for one workgroup");
@@ -893,8 +926,10 @@ void als_solve_device(const double* n12, const double* a12g, double* a1r, double
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attr = true;
   }
-  hipLaunchKernelGGL(als_kernel, dim3(1), dim3(ALS_NT), lds, s, n12, a12g, a1r, a2r, nD, D, d,
-                     niter, tol, eps, method, gwork, out_cost, out_info);
+  const int64_t n12_stride = (int64_t)nD * nD * nD * nD;
+  hipLaunchKernelGGL(als_kernel, dim3(count), dim3(ALS_NT), lds, s, n12, a12g, a1r, a2r, nD, D, d,
+                     niter, tol, eps, method, gwork, out_cost, out_info, n12_stride,
+                     (int64_t)als_work_elems(nD, D, d));
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

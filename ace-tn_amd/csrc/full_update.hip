@@ -197,6 +197,40 @@ __global__ void plain_start_kernel(const double* __restrict__ U, const double* _
   }
 }
 
+// batched form: workgroup b reads a12g of problem b and writes M, then the
+// plain start's a1r/a2r from that problem's U, s, V
+__global__ void plain_matrix_batch_kernel(const double* __restrict__ a12g, int nD, int pD,
+                                          double* __restrict__ M) {
+  const int n = nD * pD;
+  const int64_t b = blockIdx.x;
+  a12g += b * n * n;
+  M += b * n * n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    // M[(y,p)][(x,q)] = a12g[y][x][p][q]  ("yxpq->ypxq")
+    const int y = e / (pD * n), p = (e / n) % pD, x = (e % n) / pD, q = e % pD;
+    M[e] = a12g[((y * nD + x) * pD + p) * pD + q];
+  }
+}
+
+__global__ void plain_start_batch_kernel(const double* __restrict__ U,
+                                         const double* __restrict__ s,
+                                         const double* __restrict__ V, int nD, int bD, int pD,
+                                         double* __restrict__ a1r, double* __restrict__ a2r) {
+  const int n = nD * pD;
+  const int64_t b = blockIdx.x;
+  U += b * n * n;
+  V += b * n * n;
+  s += b * n;
+  a1r += b * nD * bD * pD;
+  a2r += b * nD * bD * pD;
+  for (int e = threadIdx.x; e < nD * bD * pD; e += blockDim.x) {
+    const int y = e / (bD * pD), u = (e / pD) % bD, p = e % pD;
+    const double sc = sqrt(s[u] / s[0]);
+    a1r[e] = U[(y * pD + p) * n + u] * sc;
+    a2r[e] = V[(y * pD + p) * n + u] * sc;
+  }
+}
+
 void plain_start(Ctx& c, const double* a12g, const FuDims& f, double* a1r, double* a2r) {
   const int64_t D = f.D, d = f.d, nD = f.nD, n = nD * d;
   double* M = c.take((size_t)n * n);
@@ -544,6 +578,56 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
       if (host_sweeps) *host_sweeps = hi[0];
       if (host_cost) *host_cost = hc;
       if (host_info) *host_info = hi[1];
+    });
+  });
+}
+
+size_t acetn_hip_als_batch_workspace(int64_t count, int64_t D, int64_t d) {
+  size_t r = 0;
+  guarded([&] {
+    const FuDims f = fu_dims(D, d);
+    ACETN_REQUIRE(count >= 1 && count <= (int64_t(1) << 24), "als_batch: count out of range");
+    const int64_t n = f.nD * f.d;
+    r = measure([&](Ctx& c) {
+      c.take((size_t)count * (3 * n * n + n));
+      c.take((size_t)count * als_work_elems((int)f.nD, (int)f.D, (int)f.d));
+    });
+  });
+  return r;
+}
+
+int acetn_hip_als_solve_batch(const double* n12, const double* a12g, double* a1r, double* a2r,
+                              int64_t count, int64_t D, int64_t d, int niter, double tol,
+                              int method, double epsilon, int plain_start, double* dev_cost,
+                              int32_t* dev_info, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    const FuDims f = fu_dims(D, d);
+    ACETN_REQUIRE(n12 && a12g && a1r && a2r && dev_cost && dev_info, "als_batch: null pointer");
+    ACETN_REQUIRE(count >= 1 && count <= (int64_t(1) << 24), "als_batch: count out of range");
+    ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY || method == ACETN_HIP_ALS_PINV,
+                  "als_batch: unknown method");
+    ACETN_REQUIRE(niter >= 0, "als_batch: negative niter");
+    const int64_t n = f.nD * f.d;
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      double* M = c.take((size_t)count * n * n);
+      double* U = c.take((size_t)count * n * n);
+      double* V = c.take((size_t)count * n * n);
+      double* sv = c.take((size_t)count * n);
+      double* gw = c.take((size_t)count * als_work_elems((int)f.nD, (int)f.D, (int)f.d));
+      if (!c.launch) return;
+      if (plain_start) {
+        hipLaunchKernelGGL(plain_matrix_batch_kernel, dim3(count), dim3(256), 0, c.st, a12g,
+                           (int)f.nD, (int)f.d, M);
+        ACETN_HIP_CHECK(hipGetLastError());
+        // U and V share one stride so that svd_small_batch can offset both
+        svd_small_batch(M, (int)n, (int)n, U, sv, V, (int)count, n * n, c.st);
+        hipLaunchKernelGGL(plain_start_batch_kernel, dim3(count), dim3(256), 0, c.st, U, sv, V,
+                           (int)f.nD, (int)f.D, (int)f.d, a1r, a2r);
+        ACETN_HIP_CHECK(hipGetLastError());
+      }
+      als_solve_device(n12, a12g, a1r, a2r, (int)f.nD, (int)f.D, (int)f.d, niter, tol, epsilon,
+                       gw, dev_cost, dev_info, c.st, method == ACETN_HIP_ALS_PINV ? 1 : 0,
+                       (int)count);
     });
   });
 }

@@ -313,6 +313,22 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
                         int64_t* host_sweeps, double* host_cost, int32_t* host_info,
                         void* workspace, size_t ws_bytes, void* stream);
 
+/* ``count`` independent ALS problems in one launch (one workgroup per
+ * problem; the ALS loop of als_solver.py:55-82 in each), for throughput work
+ * that has many bonds' problems at hand -- the learned-start training data,
+ * its evaluation, parameter sweeps.  No counterpart in the reference, which
+ * solves one bond at a time.  Problem b's n12 (nD^4), a12g (nD^2 d^2), a1r
+ * and a2r (nD D d) follow problem b-1's contiguously.  With plain_start != 0,
+ * a1r/a2r are first overwritten with the plain start (als_solver.py:117-146);
+ * else they hold the starts.  Results stay on the device: dev_cost[b] the
+ * final cost, dev_info[2b] the sweep count, dev_info[2b+1] the flags of
+ * acetn_hip_als_solve.  Nothing synchronises. */
+size_t acetn_hip_als_batch_workspace(int64_t count, int64_t D, int64_t d);
+int acetn_hip_als_solve_batch(const double* n12, const double* a12g, double* a1r, double* a2r,
+                              int64_t count, int64_t D, int64_t d, int niter, double tol,
+                              int method, double epsilon, int plain_start, double* dev_cost,
+                              int32_t* dev_info, void* workspace, size_t ws_bytes, void* stream);
+
 /* acetn_hip_als_solve without a workspace argument, for callers that cannot
  * allocate device memory (code under stream capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its

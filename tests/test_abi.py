@@ -61,3 +61,21 @@ def test_ctmrg_graph_config_raises_while_switched_off():
         graph = True
     with pytest.raises(NotImplementedError, match="switched off"):
         ctmrg(None, C())
+
+
+def test_learned_start_network_save_load_roundtrip(tmp_path):
+    """MLP.save writes plain tensors that load with weights_only=True; the
+    layer sizes are checked (host-side, no kernels)."""
+    import torch
+    from acetn.evolution.learned_start import MLP
+    m = MLP(40, 12, hidden=16, seed=3, device="cpu")
+    p = tmp_path / "net.pt"
+    m.save(str(p))
+    st = torch.load(str(p), weights_only=True)
+    assert st["fin"] == 40 and st["hidden"] == 16 and st["fout"] == 12
+    m2 = MLP.load(str(p), device="cpu")
+    assert torch.equal(m2.params, m.params)
+    st["params"] = st["params"][:-1]
+    torch.save(st, str(p))
+    with pytest.raises(ValueError, match="parameter count"):
+        MLP.load(str(p), device="cpu")

@@ -348,3 +348,34 @@ def test_ipeps_evolve_api_matches_oracle(hip):
     print(f"Ipeps.evolve: hip {e_h:.15f} oracle {e_o:.15f} fixture {e_ref:.15f}")
     assert e_h == pytest.approx(e_o, rel=1e-10)
     assert e_h == pytest.approx(e_ref, rel=1e-4)   # reference criterion, test_ground_states.py:33-35
+
+
+@pytest.mark.parametrize("method", ["cholesky", "pinv"])
+def test_als_batch_equals_single_solves(hip, method):
+    """acetn_hip_als_solve_batch (one workgroup per problem) against one
+    acetn_hip_als_initialize + acetn_hip_als_solve per problem: the same
+    kernels on the same data, so bit-equal results; with given starts too."""
+    probs = [als_problem(b) for b in range(8)]
+    shape = probs[0][2]
+    n12 = torch.stack([dev(p[0]) for p in probs])
+    a12g = torch.stack([dev(p[1]) for p in probs])
+    b1, b2, sw, cost, fl = hip.als_solve_batch(n12, a12g, shape[1], 100, 1e-12, method, 1e-12)
+    torch.cuda.synchronize()
+    for i in range(len(probs)):
+        s1, s2 = hip.als_initialize(a12g[i], shape[1])
+        h1, h2, hs, hc, hf = hip.als_solve(n12[i], a12g[i], s1, s2, 100, 1e-12, method, 1e-12)
+        assert int(sw[i]) == hs and float(cost[i]) == hc and int(fl[i]) == hf, i
+        assert torch.equal(b1[i], h1) and torch.equal(b2[i], h2), i
+    # from given starts (the plain starts, perturbed)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    st = [hip.als_initialize(a12g[i], shape[1]) for i in range(len(probs))]
+    s1 = torch.stack([s[0] for s in st])
+    s2 = torch.stack([s[1] for s in st])
+    s1 = s1 + 1e-3 * torch.randn(s1.shape, generator=g, device="cuda", dtype=torch.float64)
+    c1, c2, sw2, cost2, _ = hip.als_solve_batch(n12, a12g, shape[1], 100, 1e-12, method, 1e-12,
+                                                starts=(s1, s2))
+    for i in range(len(probs)):
+        h1, h2, hs, hc, _ = hip.als_solve(n12[i], a12g[i], s1[i], s2[i], 100, 1e-12, method, 1e-12)
+        assert int(sw2[i]) == hs and float(cost2[i]) == hc, i
+        assert torch.equal(c1[i], h1) and torch.equal(c2[i], h2), i
+    print(f"{method}: sweeps {sw.tolist()} (perturbed starts {sw2.tolist()})")

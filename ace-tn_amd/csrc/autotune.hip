@@ -120,12 +120,12 @@ void autotune_save(const char* path) {
   std::lock_guard<std::mutex> g(t.mu);
   FILE* f = std::fopen(path, "w");
   ACETN_REQUIRE(f != nullptr, std::string("autotune: cannot write ") + path);
-  std::fprintf(f, "# acetn gemm tile table: kind M N K flags wm wn ks ms\n");
+  std::fprintf(f, "# acetn gemm tile table: kind M N K flags wm wn ks ms variant\n");
   for (const auto& kv : t.table) {
     const auto& k = kv.first;
     const TuneChoice& c = kv.second;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %.6f\n", std::get<0>(k), std::get<1>(k),
-                 std::get<2>(k), std::get<3>(k), std::get<4>(k), c.wm, c.wn, c.ks, c.ms);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %.6f %d\n", std::get<0>(k), std::get<1>(k),
+                 std::get<2>(k), std::get<3>(k), std::get<4>(k), c.wm, c.wn, c.ks, c.ms, c.var);
   }
   std::fclose(f);
 }
@@ -140,15 +140,21 @@ int64_t autotune_load(const char* path) {
   int64_t n = 0;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
-    int kind, M, N, K, flags, wm, wn, ks;
+    int kind, M, N, K, flags, wm, wn, ks, var = 0;
     double ms = 0.0;
-    if (std::sscanf(line, "%d %d %d %d %d %d %d %d %lf", &kind, &M, &N, &K, &flags, &wm, &wn, &ks,
-                    &ms) < 8)
+    if (std::sscanf(line, "%d %d %d %d %d %d %d %d %lf %d", &kind, &M, &N, &K, &flags, &wm, &wn,
+                    &ks, &ms, &var) < 8)
       continue;
     const bool tile_ok = (wm == 1 && wn == 5) || (wm == 2 && wn == 5) || (wm == 2 && wn == 2) ||
                          (wm == 4 && wn == 4);
-    if (!tile_ok || ks < 1 || ks > 64) continue;
-    t.table[std::make_tuple(kind, M, N, K, flags)] = TuneChoice{wm, wn, ks, ms};
+    // the wider FP64 variants exist for single-stride products: 2 x 5 tiles
+    // in variants 1-3, 1 x 5 tiles in variant 3
+    const bool var_ok = var == 0 || (kind == 0 && wn == 5 && (flags & 1) &&
+                                     ((wm == 2 && var <= 3) || (wm == 1 && var == 3)));
+    if (!tile_ok || !var_ok || ks < 1 || ks > 64) continue;
+    TuneChoice c{wm, wn, ks, ms};
+    c.var = var;
+    t.table[std::make_tuple(kind, M, N, K, flags)] = c;
     ++n;
   }
   std::fclose(f);

@@ -100,9 +100,11 @@ constexpr int BK = 16, NT = 256;
 
 __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; }
 
-// Workgroup tile BM x BN with BM = 32*WM, BN = 32*WN: 4 waves in a 2 x 2
-// grid, each wave owning WM x WN MFMA blocks of 16 x 16.  Double-buffered LDS
-// (one barrier per k-tile) with the next tile prefetched into registers.
+// Workgroup tile BM x BN with BM = 16*WM*WGM, BN = 32*WN: WGM x 2 waves
+// (WGM = 2 or 4), each wave owning WM x WN MFMA blocks of 16 x 16, k-tiles
+// of BKT (16 or 32).  Double-buffered LDS (one barrier per k-tile) with the
+// next tile prefetched into registers; more waves or longer k-tiles give the
+// global loads more MFMA work to hide behind.
 // LIN: every operand map is a single stride (no grouped digits), so an
 // element's address is base + row*stride + k*stride: rows past the edge are
 // clamped to the last row (their results are never stored) and only the K
@@ -110,12 +112,18 @@ __device__ __forceinline__ bool contiguous(const IdxMap& m) { return m.s0 == 1; 
 // Otherwise the k part of the address (a mixed-radix map) is evaluated once
 // per k-tile and operand by 2*BK threads into a double-buffered LDS table
 // that every element load reads, instead of once per element.
-template <int WM, int WN, bool LIN>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, GemmBatch bt, int ksplit,
-                                                      double* __restrict__ work) {
-  constexpr int BM = 32 * WM, BN = 32 * WN;
+// Four-wave tiles up to 2 x 5 blocks per wave are held to 256 VGPRs, so two
+// workgroups share a CU (their LDS fits twice); larger tiles and the
+// eight-wave variants take the registers they need.
+template <int WM, int WN, bool LIN, int WGM = 2, int BKT = BK>
+__global__ __launch_bounds__(128 * WGM, (WGM == 2 && WM < 4 && BKT == 16) ? 2 : 1) void
+gemm_f64_kernel(GemmDesc g, GemmBatch bt, int ksplit, double* __restrict__ work) {
+  constexpr int NT = 128 * WGM;
+  constexpr int BK = BKT;
+  constexpr int BM = 16 * WM * WGM, BN = 32 * WN;
   constexpr int LA = BM + 2, LB = BN + 2;
   constexpr int AE = BM * BK / NT, BE = BN * BK / NT;  // elements per thread
+  static_assert(AE * NT == BM * BK && BE * NT == BN * BK, "tile not divisible by the threads");
   __shared__ double As[2][BK][LA];
   __shared__ double Bs[2][BK][LB];
   __shared__ int64_t koff[2][2][BK];  // [table][operand A / B][k within the tile]
@@ -388,7 +396,9 @@ __global__ __launch_bounds__(kDotNT) void dot_small_reduce(GemmDesc g, GemmBatch
 
 struct Tile {
   int wm, wn;
+  int var = 0;  // TuneChoice::var
 };
+int tile_bm(const Tile& t) { return 16 * t.wm * ((t.var & 1) ? 4 : 2); }
 // per-shape tile: skinny range-finder products (N <= 160) use 64 x 160 / 32 x 160,
 // large products 128 x 128, small ones 64 x 64
 Tile choose_tile(const GemmDesc& g) {
@@ -400,7 +410,7 @@ Tile choose_tile(const GemmDesc& g) {
 }
 
 int choose_ksplit_for(const GemmDesc& g, int count, const Tile& t) {
-  const int BM = 32 * t.wm, BN = 32 * t.wn;
+  const int BM = tile_bm(t), BN = 32 * t.wn;
   const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * count;
   if (tiles >= 512 || g.K < 4 * BK * 8) return 1;
   int ks = (int)((1024 + tiles - 1) / tiles);
@@ -421,6 +431,17 @@ void launch_gemm(const GemmDesc& g, const GemmBatch& b, int ks, double* work, hi
   const bool lin = g.am.n0 <= 0 && g.ak.n0 <= 0 && g.bk.n0 <= 0 && g.bn.n0 <= 0;
   if (lin) hipLaunchKernelGGL((gemm_f64_kernel<WM, WN, true>), grid, dim3(NT), 0, s, g, b, ks, work);
   else hipLaunchKernelGGL((gemm_f64_kernel<WM, WN, false>), grid, dim3(NT), 0, s, g, b, ks, work);
+}
+
+// the wider WM x 5 variants (single-stride operand maps only)
+template <int WM, int WGM, int BKT>
+void launch_gemm_wide(const GemmDesc& g, const GemmBatch& b, int ks, double* work,
+                      hipStream_t s) {
+  constexpr int BM = 16 * WM * WGM, BN = 32 * 5;
+  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, b.count * ks);
+  ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_f64: grid too large");
+  hipLaunchKernelGGL((gemm_f64_kernel<WM, 5, true, WGM, BKT>), grid, dim3(128 * WGM), 0, s, g, b,
+                     ks, work);
 }
 }  // namespace
 
@@ -454,9 +475,14 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
   auto fits = [&](int k) {
     return k == 1 || (work != nullptr && work_elems >= (size_t)k * b.count * g.M * g.N);
   };
+  const bool lin = g.am.n0 <= 0 && g.ak.n0 <= 0 && g.bk.n0 <= 0 && g.bn.n0 <= 0;
   auto launch_choice = [&](const TuneChoice& c) {
     hipEvent_t e0 = kernel_timing_begin(s);
-    if (c.wm == 2 && c.wn == 5) launch_gemm<2, 5>(g, b, c.ks, work, s);
+    if (c.wm == 2 && c.wn == 5 && lin && c.var == 1) launch_gemm_wide<2, 4, 16>(g, b, c.ks, work, s);
+    else if (c.wm == 2 && c.wn == 5 && lin && c.var == 2) launch_gemm_wide<2, 2, 32>(g, b, c.ks, work, s);
+    else if (c.wm == 2 && c.wn == 5 && lin && c.var == 3) launch_gemm_wide<2, 4, 32>(g, b, c.ks, work, s);
+    else if (c.wm == 1 && c.wn == 5 && lin && c.var == 3) launch_gemm_wide<1, 4, 32>(g, b, c.ks, work, s);
+    else if (c.wm == 2 && c.wn == 5) launch_gemm<2, 5>(g, b, c.ks, work, s);
     else if (c.wm == 1 && c.wn == 5) launch_gemm<1, 5>(g, b, c.ks, work, s);
     else if (c.wm == 4 && c.wn == 4) launch_gemm<4, 4>(g, b, c.ks, work, s);
     else launch_gemm<2, 2>(g, b, c.ks, work, s);
@@ -471,19 +497,24 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
       ACETN_HIP_CHECK(hipGetLastError());
     }
   };
-  const bool lin = g.am.n0 <= 0 && g.ak.n0 <= 0 && g.bk.n0 <= 0 && g.bn.n0 <= 0;
   const TuneKey key{0, g.M, g.N, g.K, (lin ? 1 : 0) | (b.count << 1)};
   if (autotune_should_tune(key, g, s)) {
     std::vector<TuneChoice> cands;
-    const Tile tiles[4] = {{1, 5}, {2, 5}, {2, 2}, {4, 4}};
+    const Tile tiles[8] = {{1, 5},    {2, 5},    {2, 2},    {4, 4},
+                           {2, 5, 1}, {2, 5, 2}, {2, 5, 3}, {1, 5, 3}};
     for (const Tile& t : tiles) {
+      if (t.var != 0 && !lin) continue;
       const int h = choose_ksplit_for(g, b.count, t);
       const int opts[4] = {1, h, 2 * h, h / 2};
       for (int k : opts) {
-        if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < BK)) continue;
+        if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < 2 * BK)) continue;
         bool dup = false;
-        for (const TuneChoice& c : cands) dup |= c.wm == t.wm && c.wn == t.wn && c.ks == k;
-        if (!dup) cands.push_back(TuneChoice{t.wm, t.wn, k});
+        for (const TuneChoice& c : cands)
+          dup |= c.wm == t.wm && c.wn == t.wn && c.var == t.var && c.ks == k;
+        if (dup) continue;
+        TuneChoice c{t.wm, t.wn, k};
+        c.var = t.var;
+        cands.push_back(c);
       }
     }
     autotune_run(key, cands, s, launch_choice);
@@ -494,7 +525,7 @@ void gemm_f64(const GemmDesc& g, const GemmBatch& b, double* work, size_t work_e
   const Tile t = choose_tile(g);
   TuneChoice ch{t.wm, t.wn, ks};
   TuneChoice tuned;
-  if (autotune_lookup(key, &tuned) && fits(tuned.ks)) ch = tuned;
+  if (autotune_lookup(key, &tuned) && fits(tuned.ks) && (tuned.var == 0 || lin)) ch = tuned;
   launch_choice(ch);
 }
 

@@ -146,6 +146,11 @@ struct TuneKey {
 struct TuneChoice {
   int wm, wn, ks;
   double ms = 0.0;
+  // FP64 kernel variant: 0 four waves (2 x 2) with k-tiles of 16, 1 eight
+  // waves (4 x 2) with k-tiles of 16, 2 four waves with k-tiles of 32,
+  // 3 eight waves with k-tiles of 32 (the split-bf16 kernels have one);
+  // the workgroup tile is (16 wm waves_m) x (32 wn)
+  int var = 0;
 };
 void autotune_set_mode(int mode);  // 0 thresholds, 1 tune new shapes, 2 table only
 int autotune_mode();

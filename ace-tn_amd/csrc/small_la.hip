@@ -736,6 +736,9 @@ __global__ __launch_bounds__(64 * kRepWaves) void jacobi_vectors_kernel(
 constexpr int kBjMaxQ = 160;
 constexpr int kBjMaxPairs = kBjMaxQ / 16;   // block pairs per block round (N / 2)
 constexpr int BJ_NT = 64 * kBjMaxPairs;
+// the eigensolver kernel itself runs 16 waves: phase 1 uses one wave per
+// block pair, phase 2 spreads the off-diagonal tiles over all 16
+constexpr int BJE_NT = 1024;
 constexpr int BJ_LD = 17;
 constexpr int kBjMaxSweeps = 40;
 
@@ -748,7 +751,7 @@ __device__ __forceinline__ int bj_global(int li, int I, int J) {
   return li < 8 ? 8 * I + li : 8 * J + li - 8;
 }
 
-__global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restrict__ Gin, int q,
+__global__ __launch_bounds__(BJE_NT) void jacobi_blk_kernel(const double* __restrict__ Gin, int q,
                                                            double* __restrict__ w,
                                                            double* __restrict__ vrec, int* info,
                                                            double tol) {
@@ -764,9 +767,9 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
   const int NP = N / 2;
   const int nstore = N * (N + 1) / 2 * 64;
   double* vflag = vrec + (int64_t)kBjMaxSweeps * (N - 1) * NP * 256;
-  for (int e = tid; e < nstore; e += BJ_NT) S[e] = 0.0;
+  for (int e = tid; e < nstore; e += BJE_NT) S[e] = 0.0;
   __syncthreads();
-  for (int e = tid; e < q * q; e += BJ_NT) {
+  for (int e = tid; e < q * q; e += BJE_NT) {
     const int i = e / q, j = e % q;
     if (i <= j) S[bj_addr(i, j)] = 0.5 * (Gin[(int64_t)i * q + j] + Gin[(int64_t)j * q + i]);
   }
@@ -785,7 +788,7 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
   int sweep = 0;
   for (; sweep < kBjMaxSweeps; ++sweep) {
     int need = 0;
-    for (int j = tid >> 3; j < q; j += BJ_NT / 8) {
+    for (int j = tid >> 3; j < q; j += BJE_NT / 8) {
       const double ajj = S[bj_addr(j, j)];
       for (int i = tid & 7; i < j && !need; i += 8) {
         const double aij = S[bj_addr(i, j)];
@@ -894,7 +897,7 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
       }
       __syncthreads();
       // ---- phase 2: S_ac <- V_a^T S_ac V_c for every pair of block pairs a < c ----
-      for (int u = wave; u < ntile; u += kBjMaxPairs) {
+      for (int u = wave; u < ntile; u += BJE_NT / 64) {
         int c = 1;
         while (c * (c + 1) / 2 <= u) ++c;
         const int a = u - c * (c - 1) / 2;
@@ -944,7 +947,7 @@ __global__ __launch_bounds__(BJ_NT) void jacobi_blk_kernel(const double* __restr
     }
   }
   if (tid == 0) info[0] = sweep;
-  for (int i = tid; i < q; i += BJ_NT) w[i] = S[bj_addr(i, i)];
+  for (int i = tid; i < q; i += BJE_NT) w[i] = S[bj_addr(i, i)];
 }
 
 // Replay of the recorded block rotations on V, one workgroup per 16-row tile
@@ -1215,7 +1218,7 @@ void jacobi_eigh(double* G, double* w, double* V, double* rot, int q, hipStream_
                                           (int)(sizeof(double) * kNmax * (kNmax + 1) / 2 * 64)));
       attr = true;
     }
-    hipLaunchKernelGGL(jacobi_blk_kernel, dim3(1), dim3(BJ_NT), lds, s, G, q, wraw, rot, info, tol);
+    hipLaunchKernelGGL(jacobi_blk_kernel, dim3(1), dim3(BJE_NT), lds, s, G, q, wraw, rot, info, tol);
     ACETN_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(jacobi_blk_vectors_kernel, dim3((8 * N + 15) / 16), dim3(BJ_NT), 0, s, rot,
                        info, q, w, V, wraw);

@@ -84,3 +84,28 @@ def test_ctmrg_iteration_with_tuning_matches_oracle(hip, cfg):
         assert e_h == pytest.approx(e_o, rel=1e-10)
     else:
         assert abs(e_h - e_o) <= 1e-6
+
+
+@pytest.mark.parametrize("wm,var", [(2, 1), (2, 2), (2, 3), (1, 3)])
+def test_wide_fp64_variants_are_exact(hip, tmp_path, wm, var):
+    """The eight-wave and k-tile-32 FP64 kernels, forced through a loaded
+    table entry (kind M N K flags wm wn ks ms variant), on a single-stride
+    range-finder shape with and without split-K: FP64-exact against numpy;
+    an entry naming a variant for a strided product is ignored on load."""
+    rng = np.random.default_rng(var)
+    M, N, K = 1040, 146, 1100
+    A, B = rng.standard_normal((M, K)), rng.standard_normal((K, N))
+    exact = A @ B
+    for ks in (1, 5):
+        path = os.path.join(tmp_path, f"t{ks}.txt")
+        with open(path, "w") as f:
+            f.write(f"0 {M} {N} {K} 3 {wm} 5 {ks} 0.0 {var}\n")
+        hip.autotune_clear()
+        assert hip.autotune_load(path) == 1
+        hip.autotune("table")
+        C = hip.gemm(dev(A), dev(B)).cpu().numpy()
+        np.testing.assert_allclose(C, exact, rtol=0, atol=1e-12 * np.abs(exact).max())
+    with open(path, "w") as f:
+        f.write(f"0 {M} {N} {K} 2 {wm} 5 1 0.0 {var}\n")   # flags 2: not single-stride
+    hip.autotune_clear()
+    assert hip.autotune_load(path) == 0

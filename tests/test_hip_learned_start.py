@@ -94,3 +94,87 @@ def test_learned_start_keeps_the_solution_and_reports_sweeps(hip):
     print(f"loss {losses[0]:.3e} -> {losses[-1]:.3e}; {res}")
     assert res["max_rel_cost_diff"] < 1e-8
     assert res["mean_sweeps_learned"] <= res["mean_sweeps_plain"] + 0.25
+
+
+def test_c4_plain_solves_sit_at_the_three_sweep_floor(hip):
+    """Why the learned start cannot cut sweeps at the C4 configuration
+    (Heisenberg, D=6, chi=72, dtau=0.01, tol 1e-10): the stopping rule
+    `error < tol and i > 1` (als_solver.py:79) makes 3 sweeps the minimum,
+    the plain start already reaches it on these problems -- from the first
+    sweep of a random start on -- and so does a start at the converged
+    solution itself.  (The 20k-problem run of start_workload gives the same
+    histogram: all 20000 plain solves take 3 sweeps.)"""
+    from acetn.evolution import start_workload as sw
+    from acetn.ipeps.ipeps_config import EvolutionConfig
+    cfg = EvolutionConfig()
+    data = sw.generate(6, 72, 0.01, [7], 2)
+    _, _, sweeps, cost, flags = hip.als_solve_batch(data['n12'], data['a12g'], 6, cfg.als_niter,
+                                                    1e-10, cfg.als_method, cfg.als_epsilon)
+    _, _, s_opt, c_opt, _ = hip.als_solve_batch(data['n12'], data['a12g'], 6, cfg.als_niter, 1e-10,
+                                                cfg.als_method, cfg.als_epsilon,
+                                                starts=(data['a1r'], data['a2r']))
+    torch.cuda.synchronize()
+    print(f"{sweeps.numel()} problems: plain sweeps {sw.histogram(sweeps)}, "
+          f"from the solution {sw.histogram(s_opt)}")
+    assert sweeps.numel() == 16 and not flags.any()
+    assert bool((sweeps == 3).all()) and bool((s_opt == 3).all())
+    rel = ((cost - c_opt).abs() / c_opt.abs()).max()
+    assert float(rel) < 1e-9
+
+
+def test_als_start_learned_through_the_config(hip, tmp_path):
+    """evolution.als_start='learned' with evolution.als_start_model: evolve()
+    loads the saved network and every bond's ALS goes through the proposer
+    (the plain start judging it); the energy agrees with the plain run to the
+    ALS tolerance."""
+    from acetn.evolution.evolve import evolve
+    from acetn.evolution.learned_start import (MLP, dataset, feature_sizes, generate_problems,
+                                               train)
+    from acetn.ipeps import Ipeps
+    from acetn.model.model_factory import model_factory
+    st, _, D, chi = load_golden("heisenberg_dims_3_16_dtau_001")
+
+    def net(start, path=None):
+        ev = {'disable_progressbar': True}
+        if start == "learned":
+            ev.update(als_start="learned", als_start_model=str(path))
+        cfg = {'dtype': 'float64', 'device': 'cuda',
+               'TN': {'nx': 2, 'ny': 2, 'dims': {'phys': 2, 'bond': D, 'chi': chi}},
+               'model': {'name': 'heisenberg', 'params': {'J': 1.0}},
+               'ctmrg': {'steps': 2, 'sketch_seed': 3, 'disable_progressbar': True},
+               'evolution': ev}
+        ipeps = Ipeps(cfg)
+        for s in ipeps.site_list:
+            ipeps[s]['A'] = torch.as_tensor(st[s]['A'], device="cuda")
+            ipeps[s]['C'] = [torch.as_tensor(c, device="cuda") for c in st[s]['C']]
+            ipeps[s]['E'] = [torch.as_tensor(e, device="cuda") for e in st[s]['E']]
+        ipeps.site_states_initialized = True
+        return ipeps
+
+    train_net = net("plain")
+    model = model_factory.create(train_net.config.model)
+    probs = generate_problems(train_net, model, 0.01, 4, train_net.config.evolution)
+    fin, fout = feature_sizes(2 * D, 2)
+    mlp = MLP(fin, fout, hidden=128, seed=0)
+    X, T = dataset(probs)
+    train(mlp, X, T, epochs=10, batch=16, lr=1e-3)
+    path = tmp_path / "start.pt"
+    mlp.save(str(path))
+    energies = {}
+    for start in ("plain", "learned"):
+        ipeps = net(start, path)
+        upd = evolve(ipeps, 0.01, 2, model, ipeps.config.evolution)
+        if start == "learned":
+            p = upd.proposer
+            assert p is not None and p.used + p.rejected == 2 * len(ipeps.bond_list)
+            print(f"proposals used {p.used}, rejected {p.rejected}")
+        else:
+            assert upd.proposer is None
+        ipeps.renormalize()
+        energies[start] = float(ipeps.measure()['Energy'])
+    print(energies)
+    assert energies["learned"] == pytest.approx(energies["plain"], rel=1e-7)
+    bad = net("learned", tmp_path / "missing.pt")
+    bad.config.evolution.als_start_model = None
+    with pytest.raises(ValueError, match="als_start_model"):
+        evolve(bad, 0.01, 1, model, bad.config.evolution)

@@ -207,7 +207,15 @@ def run_ctmrg(precision, args, A, keep_state=False):
         log(f"[{precision}] warmup {i}: C0 {tuple(net[(0, 0)]['C'][0].shape)}")
     torch.cuda.synchronize()
     mode = args.autotune if precision == "fp64" else args.autotune_mixed
-    if mode != "off":
+    if args.tile_table:
+        # a saved tile table (e.g. from an earlier run's --save-tile-table):
+        # the same kernel choices without a tuning iteration, as profiling
+        # runs need
+        _hip.autotune_clear()
+        n = _hip.autotune_load(args.tile_table)
+        _hip.autotune("table")
+        log(f"[{precision}] loaded {n} tile choices from {args.tile_table}")
+    elif mode != "off":
         # one more iteration tunes the tile shape of each product shape (with
         # the projector pairs serialised, every product timed alone, or
         # concurrent as in the timed loop); the timed iterations then use the
@@ -222,6 +230,8 @@ def run_ctmrg(precision, args, A, keep_state=False):
         _hip.autotune("table")
         log(f"[{precision}] autotuned {_hip.autotune_entries()} product shapes in "
             f"{time.perf_counter() - t0:.1f} s")
+        if args.save_tile_table:
+            _hip.autotune_save(f"{args.save_tile_table}.{precision}")
     warm = None
     if keep_state:
         warm = {s: {'A': net[s]['A'].cpu().numpy(), 'C': [c.cpu().numpy() for c in net[s]['C']],
@@ -319,7 +329,9 @@ def run_als(args):
     res["at_default_tol"] = {"tol": cfg.als_tol, **ref_tol,
                              "sweep_reduction": 1.0 - ref_tol["mean_sweeps_learned"] /
                              ref_tol["mean_sweeps_plain"]}
-    # plain ALS solves/s (plain start included), to rel. cost change < 1e-10
+    # plain ALS solves/s (plain start included), to rel. cost change < 1e-10:
+    # one bond at a time (the full update's order, a host readback each) and
+    # all problems in one batched launch (one workgroup per problem)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for pr in test_p:
@@ -327,6 +339,17 @@ def run_als(args):
         solve_from(pr['n12'], pr['a12g'], a1, a2, cfg, 1e-10)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    n12s = torch.stack([pr['n12'] for pr in probs])
+    a12s = torch.stack([pr['a12g'] for pr in probs])
+    _hip.als_solve_batch(n12s, a12s, D, cfg.als_niter, 1e-10, cfg.als_method, cfg.als_epsilon)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, sw_b, _, _ = _hip.als_solve_batch(n12s, a12s, D, cfg.als_niter, 1e-10, cfg.als_method,
+                                            cfg.als_epsilon)
+    torch.cuda.synchronize()
+    dt_batch = time.perf_counter() - t0
+    vals, cnts = torch.unique(sw_b.cpu(), return_counts=True)
+    hist = {int(v): int(c) for v, c in zip(vals, cnts)}
     # the FP64 oracle's ALS on the host, same problems and start, first 16
     cpu = None
     if not args.no_cpu_baseline:
@@ -342,7 +365,11 @@ def run_als(args):
                "cores": len(os.sched_getaffinity(0)), "kind": "port",
                "sample": f"{len(sample)} of the test problems, plain start, rel. cost change "
                          f"< 1e-10, FP64 numpy oracle"}
-    res.update({"als_solves_per_s": len(test_p) / dt, "train_problems": ntr,
+    res.update({"als_solves_per_s": len(test_p) / dt,
+                "als_solves_per_s_batched": len(probs) / dt_batch,
+                "batched_problems": len(probs),
+                "plain_sweep_histogram_all_problems": hist,
+                "train_problems": ntr,
                 "cpu_baseline": cpu,
                 "train_loss_first_last": [losses[0], losses[-1]],
                 "sweep_reduction": 1.0 - res["mean_sweeps_learned"] / res["mean_sweeps_plain"],
@@ -482,6 +509,10 @@ def main():
                     help="run the timed iterations with the projector pairs serialised")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="run one precision only (profiling runs); prints no JSON line")
+    ap.add_argument("--tile-table", default=None, metavar="PATH",
+                    help="load this tile table instead of tuning (both precisions)")
+    ap.add_argument("--save-tile-table", default=None, metavar="PREFIX",
+                    help="write each precision's tuned table to PREFIX.<precision>")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB once with a plain device copy (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -520,6 +551,7 @@ def main():
     import copy as _copy
     alt = _copy.copy(args)
     alt.autotune = "off" if args.autotune != "off" else "serial"
+    alt.tile_table = None
     ref2 = run_ctmrg("fp64", alt, A)
     spread = abs(ref2["energy"] - ref["energy"])
     # Roofline of the matrix-core class with the most event time in the

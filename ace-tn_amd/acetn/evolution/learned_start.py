@@ -4,8 +4,9 @@ A 3-layer perceptron (hidden width 512 by default, bf16 matrix-core
 products with FP32 accumulation, hand-written forward / backward / Adam
 kernels in libacetn_hip) maps one ALS problem of the full update -- the
 gauge-fixed norm tensor n12[y][x][Y][X] and the gated bond tensor
-a12g[y][x][p][q] -- to a correction of the plain start's bond product
-a12p = a1r a2r (the truncated SVD of a12g, als_solver.py:117-146):
+a12g[y][x][p][q] -- to a correction of a12p, the rank-D truncation of a12g
+(the plain start's bond product a1r a2r, als_solver.py:117-146, which divides
+the singular values by the largest, times that largest value):
 
     a12_proposed = a12p + ||a12g|| * MLP(features)
 
@@ -46,11 +47,19 @@ def features(n12, a12g, a12p):
     return torch.cat([p.reshape(-1) for p in parts]).to(torch.float32)
 
 
+def rescaled(a12p, a12g):
+    """The plain start's product scaled to a12g: the start divides the
+    singular values by the largest one (als_solver.py:140), so its product is
+    the rank-D truncation of a12g over s_0; alpha = <a12p, a12g> / |a12p|^2
+    restores the truncation itself."""
+    return a12p * ((a12p * a12g).sum() / (a12p * a12p).sum())
+
+
 def plain_product(a12g):
-    """The plain start's bond product a12p (device)."""
+    """The rank-D truncation of a12g from the plain start (device)."""
     nD, _, d, _ = a12g.shape
     p1, p2 = _hip.als_initialize(a12g, nD // d)
-    return bond_product(p1, p2)
+    return rescaled(bond_product(p1, p2), a12g)
 
 
 def bond_product(a1r, a2r):
@@ -153,7 +162,7 @@ class MLPProposer:
     def choose(self, n12, a12g, a1r, a2r):
         """(a1r, a2r) is the plain start."""
         bD = a1r.shape[1]
-        a12p = bond_product(a1r, a2r)
+        a12p = rescaled(bond_product(a1r, a2r), a12g)
         p1, p2 = _hip.als_initialize(self.propose(n12, a12g, a12p).contiguous(), bD)
         if start_cost(n12, a12g, p1, p2, self.config) <= start_cost(n12, a12g, a1r, a2r,
                                                                     self.config):

@@ -83,12 +83,12 @@ def histogram(sweeps):
 
 def proposals(mlp, n12, a12g, p1, p2, chunk=2048):
     """a12p + ||a12g|| MLP(features) for every problem (device), a12p the
-    bond product of the plain start (p1, p2)."""
-    from .learned_start import bond_product, features
+    rank-D truncation of a12g from the plain start (p1, p2)."""
+    from .learned_start import bond_product, features, rescaled
     out = torch.empty_like(a12g)
     for i in range(0, a12g.shape[0], chunk):
         js = range(i, min(i + chunk, a12g.shape[0]))
-        a12p = [bond_product(p1[j], p2[j]) for j in js]
+        a12p = [rescaled(bond_product(p1[j], p2[j]), a12g[j]) for j in js]
         X = torch.stack([features(n12[j], a12g[j], a12p[k]) for k, j in enumerate(js)])
         y = mlp.forward(X).to(torch.float64)
         a = a12g[i:i + chunk]

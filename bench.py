@@ -502,9 +502,10 @@ def main():
                          "tuning iteration with the projector pairs serialised / concurrent")
     ap.add_argument("--autotune-mixed", choices=["off", "serial", "concurrent"], default="serial",
                     help="the same for the mixed-precision run")
-    ap.add_argument("--energy-iters", type=int, default=0,
+    ap.add_argument("--energy-iters", type=int, default=40,
                     help="untimed iterations after the timed ones, before the energies are "
-                         "measured")
+                         "measured (the environment closer to its fixed point, where the "
+                         "reference measures)")
     ap.add_argument("--serial", action="store_true",
                     help="run the timed iterations with the projector pairs serialised")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
@@ -607,8 +608,12 @@ def main():
                             "rel_diff": dE / abs(ref["energy"]),
                             "fp64_rounding_spread": spread,
                             "fp64_other_tiles": ref2["energy"],
-                            "note": "Heisenberg J=1 bond operator on the final environments; "
-                                    "the FP64 HIP path is held to the FP64 oracle by the tests"},
+                            "iterations_before_measurement": args.warmup + 1 + args.steps + 1 +
+                            args.energy_iters,
+                            "note": "Heisenberg J=1 bond operator on the final environments, "
+                                    f"{args.energy_iters} untimed iterations after the timed "
+                                    "ones; the FP64 HIP path is held to the FP64 oracle by the "
+                                    "tests"},
         "nn_correlators": {"mixed": mix["correlators"], "fp64": ref["correlators"],
                            "max_abs_diff": max(abs(mix["correlators"][k] - ref["correlators"][k])
                                                for k in ref["correlators"])},

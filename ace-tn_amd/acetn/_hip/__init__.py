@@ -77,6 +77,12 @@ def _declare(lib):
                                                     c_i64, c_i64, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_int, ctypes.c_int, c_i64, c_vp, c_vp,
                                                     c_vp, c_size, c_vp]),
+        "acetn_hip_projectors_begin_mark": (ctypes.c_int, [ctypes.POINTER(AcetnSite) * 4,
+                                                           ctypes.c_int, c_i64, c_i64,
+                                                           ctypes.c_int, c_dbl, c_u64, c_u64,
+                                                           ctypes.c_int, ctypes.c_int,
+                                                           ctypes.c_int, c_vp, c_vp, c_vp,
+                                                           c_size, c_vp, c_vp]),
         "acetn_hip_projectors_begin_graph": (ctypes.c_int, [ctypes.POINTER(AcetnSite) * 4,
                                                             ctypes.c_int, c_i64, c_i64,
                                                             ctypes.c_int, c_dbl, c_u64, c_u64,
@@ -487,14 +493,17 @@ class ProjectorJob:
     order; system 'half-system' | 'full-system', svd 'rsvd' | 'full-rank'."""
 
     def __init__(self, slot, sites, k, chi, oversampling, niter, cutoff, seed, stream_id,
-                 precision="fp64", system="half-system", svd="rsvd", ws_key=None, graph=None):
+                 precision="fp64", system="half-system", svd="rsvd", ws_key=None, graph=None,
+                 mark=False):
         """With ``graph`` = (stream_base, counts, index) the pair is enqueued in
         the capturable form (acetn_hip_projectors_begin_graph / _end_padded):
         the sketch stream is stream_id + stream_base[0] (an int64 device
         tensor), the truncation count goes to counts[index] (an int32 device
         tensor), nothing synchronises, the workspace is private to the job,
         and ``finish`` returns min(chi, rows) columns with those past the
-        count zero."""
+        count zero.  With ``mark`` the event ``self.mark`` is recorded on the
+        pair's stream when its range-finder products are enqueued, ahead of
+        the eigensolver (acetn_hip_projectors_begin_mark)."""
         made = [make_site(*st) for st in sites]
         self.sv = [m[0] for m in made]
         self.keep = [m[1] for m in made]
@@ -530,13 +539,25 @@ class ProjectorJob:
         # until their second phase has run
         self.ws = slot_workspace(slot if ws_key is None else ("pair", ws_key), nb, self.dev)
         self.cd_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
+        self.mark = None
         with torch.cuda.stream(self.stream):
             # zero-filled on the pair's stream, ahead of the kernels that write it
             self.spec = torch.zeros(nspec, dtype=torch.float64, device=self.dev)
-            check(lib().acetn_hip_projectors_begin(
-                self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
-                self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec), ptr(self.ws),
-                self.ws.numel(), c_vp(self.stream.cuda_stream)))
+            if mark:
+                # the event object exists once recorded; the library records it
+                # again at the mark, and waits issued later refer to that record
+                self.mark = torch.cuda.Event()
+                self.mark.record(self.stream)
+                check(lib().acetn_hip_projectors_begin_mark(
+                    self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
+                    self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec),
+                    ptr(self.ws), self.ws.numel(), c_vp(self.stream.cuda_stream),
+                    c_vp(self.mark.cuda_event)))
+            else:
+                check(lib().acetn_hip_projectors_begin(
+                    self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
+                    self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec),
+                    ptr(self.ws), self.ws.numel(), c_vp(self.stream.cuda_stream)))
             # marks the truncation count's arrival in pinned host memory
             self.ready = torch.cuda.Event()
             self.ready.record(self.stream)

@@ -24,10 +24,28 @@ def _spectra_change(a, b):
     return m
 
 
-def ctmrg_iteration(mover, ipeps):
+def ctmrg_iteration(mover, ipeps, pipelined="auto"):
     """One pass of ctmrg.py:26-31 (non-distributed order).  Each
-    left/right (up/down) pair is independent and runs as one stage."""
+    left/right (up/down) pair is independent and runs as one stage; with
+    half-system projectors the left and right (up and down) moves form two
+    independent chains, which run pipelined (run_chain_pair) with the same
+    results as stage by stage.  ``pipelined``: False (stage by stage), "free"
+    (the chains without a stage barrier between them), "stagger" (each
+    chain's products started when the other chain's pairs reach their
+    eigensolvers) or "auto": staggered in mixed precision, where the
+    one-workgroup eigensolvers dominate a pair's time, free in FP64, where
+    the products do and two pairs alone fill the GPU less well than four."""
+    from .directional_mover import run_chain_pair
     nx, ny = ipeps.nx, ipeps.ny
+    if pipelined and mover.projector_calculator.projectors == "half-system":
+        if pipelined == "auto":
+            pipelined = "stagger" if mover.projector_calculator.precision == "mixed" else "free"
+        stagger = pipelined == "stagger"
+        run_chain_pair(mover, ipeps, [(xi, 0) for xi in range(nx)],
+                       [((nx - xi + 1) % nx, 2) for xi in range(nx)], stagger)
+        run_chain_pair(mover, ipeps, [((ny - yi + 1) % ny, 1) for yi in range(ny)],
+                       [(yi, 3) for yi in range(ny)], stagger)
+        return
     for xi in range(nx):
         mover.stage(ipeps, [(xi, 0), ((nx - xi + 1) % nx, 2)])
     for yi in range(ny):

@@ -221,3 +221,109 @@ def run_stages(items, capture=None):
             dst['C'][(3 + k) % 4] = Ca
             dst['C'][k] = Cb
             dst['E'][(3 + k) % 4] = Eo
+
+
+def run_chain_pair(mover, ipeps, chain_a, chain_b, stagger=True):
+    """The two independent move chains of one half of a CTMRG iteration,
+    pipelined.  ``chain_a`` / ``chain_b`` are the left (up) and right (down)
+    moves of ctmrg_iteration's stages, in stage order: the left move of
+    column x reads C0, C3, E0, E2, E3 of column x and writes C0, C3, E3 of
+    column x+1, the right move reads C1, C2, E0, E1, E2 and writes C1, C2, E1
+    (likewise up/down), so each chain depends only on its own previous move
+    and the two chains share nothing they write.
+
+    Stage k of chain a runs on slots 0-1, of chain b on slots 2-3.  Chain b's
+    pairs of stage k start their matrix-core work once chain a's pairs of that
+    stage have enqueued theirs (their mark events, recorded ahead of the
+    one-workgroup eigensolver), and chain a's pairs of stage k+1 once chain b's
+    of stage k have: each chain's latency-bound eigensolves run while the
+    other chain's products fill the GPU, where running the two moves of a
+    stage side by side puts all four eigensolves at the same time.  Every pair
+    and renormalisation computes exactly what run_stages computes (same
+    kernels, inputs and sketch streams, taken in the same order), so the
+    results are the same bit for bit.  Workspaces are keyed per chain, stage
+    and pair (each used once per call), and the tensors a chain replaces stay
+    referenced until the final synchronisation."""
+    import torch
+    calc = mover.projector_calculator
+    n = len(chain_a)
+    if len(chain_b) != n:
+        raise ValueError("run_chain_pair: chains of different length")
+    chains = (chain_a, chain_b)
+    slots = ((0, 1), (2, 3))
+    first_site = ipeps[ipeps.site_list[0]]['A']
+    dev = first_site.device
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(dev))
+    for sl in range(1, _hip.N_SLOTS):
+        _hip.slot_stream(sl, dev).wait_event(ev)
+    keep = []
+    renorm_done = [[], []]   # completion events of each chain's latest renormalisations
+    marks = [[], []]         # mark events of each chain's latest started pairs
+    state = [None, None]     # (jobs, targets, started) of each chain's stage in flight
+
+    def begin(c, k):
+        jobs, targets = mover._plan(ipeps, [chains[c][k]])
+        for sl in slots[c]:
+            st = _hip.slot_stream(sl, dev)
+            for e in renorm_done[c] + (marks[1 - c] if stagger else []):
+                st.wait_event(e)
+        js = calc.start_many(ipeps, jobs, slots=[slots[c][i % 2] for i in range(len(jobs))],
+                             ws_keys=[("chain", c, k, i) for i in range(len(jobs))],
+                             mark=stagger)
+        marks[c] = [j.mark for j in js] if stagger else []
+        state[c] = (targets, js, k)
+        keep.append(js)
+
+    def finish(c):
+        targets, js, k = state[c]
+        proj, done = [None] * len(js), [None] * len(js)
+        parts, results = [None] * len(targets), [None] * len(targets)
+        flat = lambda P: P.reshape(-1, P.shape[-1])
+        ends = []
+        for idx, pr, dn in calc.finish_each(ipeps, js):
+            proj[idx], done[idx] = pr, dn
+            for t, (s1, s2, i, j, kk) in enumerate(targets):
+                if results[t] is not None:
+                    continue
+                slot = slots[c][t % 2]
+                stream = _hip.slot_stream(slot, dev)
+                if parts[t] is None and (proj[j] is not None or proj[i] is not None):
+                    a = j if proj[j] is not None else i
+                    stream.wait_event(done[a])
+                    with torch.cuda.stream(stream):
+                        st = ipeps[s1]
+                        parts[t] = (_hip.RenormalizeParts(
+                            (st['A'], st['C'], st['E']), kk, ipeps.dims['chi'],
+                            first_pair="j" if a == j else "i", slot=slot,
+                            precision=calc.precision, ws_key=("chain", c, k, t)), a)
+                        parts[t][0].first(flat(proj[a][0]), flat(proj[a][1]))
+                if parts[t] is not None and proj[i] is not None and proj[j] is not None:
+                    b = i if parts[t][1] == j else j
+                    stream.wait_event(done[b])
+                    with torch.cuda.stream(stream):
+                        results[t] = parts[t][0].second(flat(proj[b][0]), flat(proj[b][1]))
+                        e = torch.cuda.Event()
+                        e.record(stream)
+                        ends.append(e)
+        keep.append((proj, parts, results))
+        renorm_done[c] = ends
+        # the next move of this chain reads what this one wrote
+        for (s1, s2, i, j, kk), (Ca, Cb, Eo) in zip(targets, results):
+            dst = ipeps[s2]
+            keep.append((dst['C'][(3 + kk) % 4], dst['C'][kk], dst['E'][(3 + kk) % 4]))
+            dst['C'][(3 + kk) % 4] = Ca
+            dst['C'][kk] = Cb
+            dst['E'][(3 + kk) % 4] = Eo
+
+    begin(0, 0)
+    begin(1, 0)
+    for k in range(n):
+        finish(0)
+        if k + 1 < n:
+            begin(0, k + 1)
+        finish(1)
+        if k + 1 < n:
+            begin(1, k + 1)
+    torch.cuda.synchronize()
+    del keep

@@ -70,14 +70,17 @@ class ProjectorCalculator:
         torch.cuda.synchronize()
         return [pr for _, pr, _ in out]
 
-    def start_many(self, ipeps, jobs, system=None, base=0, capture=None):
+    def start_many(self, ipeps, jobs, system=None, base=0, capture=None, slots=None,
+                   ws_keys=None, mark=False):
         """Enqueue every projector pair up to its truncation count; returns
         the started jobs in ``jobs`` order.  Pair i is round-wide pair
         base + i (``base`` counts the pairs of networks started before it in
         the same round) and runs on slot (base + i) % N_SLOTS with a
         workspace of its own.  Under ``capture`` (a graph capture in
         progress) the pairs are enqueued in the capturable form, their
-        sketch streams numbered by the capture."""
+        sketch streams numbered by the capture.  ``slots`` / ``ws_keys``
+        (one per job) override the slot and workspace assignment; ``mark``
+        records each pair's mark event (ProjectorJob)."""
         system = system or ("full-system" if self.calculate == self.calculate_full_system
                             else "half-system")
         if self.svd_type not in ("rsvd", "full-rank"):
@@ -85,8 +88,8 @@ class ProjectorCalculator:
         chi = ipeps.dims['chi']
         started = []
         for i, (sites, k) in enumerate(jobs):
-            slot = (base + i) % _hip.N_SLOTS
-            ws_key = base + i
+            slot = (base + i) % _hip.N_SLOTS if slots is None else slots[i]
+            ws_key = base + i if ws_keys is None else ws_keys[i]
             sites = [tuple(s) for s in sites]
             for s in sites:
                 ipeps[s]  # raises ValueError for an unknown site
@@ -99,7 +102,8 @@ class ProjectorCalculator:
                                              chi, self.rsvd_oversampling, self.rsvd_niter,
                                              self.svd_cutoff, self.sketch.seed, stream_id,
                                              precision=self.precision, system=system,
-                                             svd=self.svd_type, ws_key=ws_key, graph=graph))
+                                             svd=self.svd_type, ws_key=ws_key, graph=graph,
+                                             mark=mark))
         return started
 
     def finish_each(self, ipeps, started):

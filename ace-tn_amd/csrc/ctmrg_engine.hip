@@ -171,6 +171,9 @@ struct HalfOut {
   const uint64_t* stream_base = nullptr;
   int32_t* dev_cD = nullptr;
   bool padded = false;
+  // phase 1: recorded on the stream once the matrix-core work of the range
+  // finder is enqueued, ahead of the one-workgroup eigensolver
+  hipEvent_t mark = nullptr;
 };
 
 // A product with an enlarged corner as its A operand: from the corner's bf16
@@ -271,6 +274,7 @@ void half_projectors(Ctx& c, const acetn_site* s1, const acetn_site* s4, int k, 
   // B^T = Q4^H (Q1^H Qm): n x q ; keep Wq = Q1^H Qm for P1
   mm_corner(c, q1t, (int)kk, (int)q, (int)m, true, Q1, kk, Y, q, Wq, q);
   mm_corner(c, q4t, (int)n, (int)q, (int)kk, true, Q4, n, Wq, q, BT, q);
+  if (out.mark && c.launch) ACETN_HIP_CHECK(hipEventRecord(out.mark, c.st));
   small_svd(c, BT, n, q, w, UB);
   if (!c.run) {
     // size the split-K scratch of the post-truncation products at cD = chi
@@ -451,6 +455,7 @@ void general_projectors(Ctx& c, const acetn_site* const* st, int k, int system, 
     }
     mm(c, (int)f, (int)a, (int)cc, true, Rp, f, true, Lp, cc, BT, a);
   }
+  if (out.mark && c.launch) ACETN_HIP_CHECK(hipEventRecord(out.mark, c.st));
   small_svd(c, BT, f, q, w, UB);
   if (!c.run) {
     const int cDp = (int)std::min<int64_t>(chi, q);
@@ -1014,12 +1019,24 @@ int acetn_hip_projectors_begin(const acetn_site* const sites[4], int k, int64_t 
                                uint64_t sketch_stream, int system, int svd_type, int precision,
                                int32_t* pinned_cD_new, double* spectrum, void* ws,
                                size_t ws_bytes, void* stream) {
+  return acetn_hip_projectors_begin_mark(sites, k, chi, oversampling, niter, svd_cutoff, seed,
+                                         sketch_stream, system, svd_type, precision,
+                                         pinned_cD_new, spectrum, ws, ws_bytes, stream, nullptr);
+}
+
+int acetn_hip_projectors_begin_mark(const acetn_site* const sites[4], int k, int64_t chi,
+                                    int64_t oversampling, int niter, double svd_cutoff,
+                                    uint64_t seed, uint64_t sketch_stream, int system,
+                                    int svd_type, int precision, int32_t* pinned_cD_new,
+                                    double* spectrum, void* ws, size_t ws_bytes, void* stream,
+                                    void* mark_event) {
   return guarded([&] {
     check_projector_args(sites, system, svd_type);
     check_precision(precision);
     ACETN_REQUIRE(pinned_cD_new, "projectors_begin: null cD target");
     ACETN_REQUIRE(niter >= 0 && niter <= 16, "projectors: niter out of range");
     HalfOut o{nullptr, nullptr, nullptr, pinned_cD_new, 0, spectrum};
+    o.mark = reinterpret_cast<hipEvent_t>(mark_event);
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       c.mixed = precision == ACETN_HIP_MIXED;
       if (system == kSysHalf && svd_type == kSvdRsvd)

@@ -176,6 +176,18 @@ int acetn_hip_projectors_end(const acetn_site* const sites[4], int k, int64_t ch
                              int precision, int64_t cD_new, double* P1, double* P2,
                              void* workspace, size_t ws_bytes, void* stream);
 
+/* acetn_hip_projectors_begin that also records ``mark_event`` (a hipEvent_t)
+ * on the stream once the range finder's matrix-core products are enqueued,
+ * ahead of the one-workgroup eigensolver (no counterpart in the reference):
+ * a caller running independent projector pairs can start another pair's
+ * products when this one reaches its latency-bound part. */
+int acetn_hip_projectors_begin_mark(const acetn_site* const sites[4], int k, int64_t chi,
+                                    int64_t oversampling, int niter, double svd_cutoff,
+                                    uint64_t seed, uint64_t sketch_stream, int system,
+                                    int svd_type, int precision, int32_t* pinned_cD_new,
+                                    double* spectrum, void* workspace, size_t ws_bytes,
+                                    void* stream, void* mark_event);
+
 /* The same two phases in a form that HIP stream capture can record (no
  * counterpart in the reference, whose loop reads cD_new on the host,
  * projectors.py:163-164; for callers that capture a whole CTMRG iteration

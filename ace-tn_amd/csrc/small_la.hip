@@ -835,6 +835,7 @@ __global__ __launch_bounds__(BJE_NT) void jacobi_blk_kernel(const double* __rest
             double c = 1.0, sn = 0.0;
             const bool live = (p < 8 ? p < nI : p - 8 < nJ) && (qq < 8 ? qq < nI : qq - 8 < nJ);
             if (live) {
+              // the three operands in one LDS round trip
               const double app = W[p * (BJ_LD + 1)], aqq = W[qq * (BJ_LD + 1)], apq = W[p * BJ_LD + qq];
               if (apq != 0.0 && apq * apq > tol * tol * fabs(app * aqq)) jacobi_rotation(app, aqq, apq, c, sn);
             }
@@ -843,6 +844,17 @@ __global__ __launch_bounds__(BJE_NT) void jacobi_blk_kernel(const double* __rest
             pqw[wave][lane] = p | (qq << 8);
           }
           rep_wave_sync();
+          // Every rotation parameter this lane needs (its W block pair and its
+          // two V pairs) is read at once, and the V operands are loaded next to
+          // the W operands: the round costs a few LDS round trips instead of
+          // one per dependent read.  Unrotated pairs hold (1, 0) and valid
+          // indices, so the loads need no branch.
+          const double c0 = csw[wave][vk][0], s0 = csw[wave][vk][1];
+          const double c1 = csw[wave][vk + 1][0], s1 = csw[wave][vk + 1][1];
+          const int pq0 = pqw[wave][vk], pq1 = pqw[wave][vk + 1];
+          double* vrow = V + vr * BJ_LD;
+          const int vp0 = pq0 & 255, vq0 = pq0 >> 8, vp1 = pq1 & 255, vq1 = pq1 >> 8;
+          const double v0p = vrow[vp0], v0q = vrow[vq0], v1p = vrow[vp1], v1q = vrow[vq1];
           if (lane < 36) {
             const double ca = csw[wave][ka][0], sa = csw[wave][ka][1];
             const double cb = csw[wave][kb][0], sb = csw[wave][kb][1];
@@ -869,16 +881,14 @@ __global__ __launch_bounds__(BJE_NT) void jacobi_blk_kernel(const double* __rest
               W[i22] = sa * y12 + ca * y22;
             }
           }
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const double c = csw[wave][vk + h][0], sn = csw[wave][vk + h][1];
-            if (sn != 0.0) {
-              const int pq = pqw[wave][vk + h];
-              double* vrow = V + vr * BJ_LD;
-              const double vp = vrow[pq & 255], vq = vrow[pq >> 8];
-              vrow[pq & 255] = c * vp - sn * vq;
-              vrow[pq >> 8] = sn * vp + c * vq;
-            }
+          // the two pairs of this lane's V row are disjoint
+          if (s0 != 0.0) {
+            vrow[vp0] = c0 * v0p - s0 * v0q;
+            vrow[vq0] = s0 * v0p + c0 * v0q;
+          }
+          if (s1 != 0.0) {
+            vrow[vp1] = c1 * v1p - s1 * v1q;
+            vrow[vq1] = s1 * v1p + c1 * v1q;
           }
           rep_wave_sync();
         }

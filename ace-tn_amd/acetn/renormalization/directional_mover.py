@@ -178,32 +178,50 @@ def run_stages(items, capture=None):
         proj, done, results = pl["proj"], pl["done"], pl["results"]
         parts = [None] * len(pl["targets"])
         flat = lambda P: P.reshape(-1, P.shape[-1])
-        for idx, pr, ev in calc.finish_each(ipeps, pl["started"]):
+        njobs = len(pl["started"])
+        job_slot = [(pl["base"] + x) % _hip.N_SLOTS for x in range(njobs)]
+
+        def clear(sl):
+            # every pair of this network on slot sl has its projector
+            # products enqueued
+            return all(proj[x] is not None for x in range(njobs) if job_slot[x] == sl)
+
+        for idx, pr, ev in calc.finish_each(ipeps, pl["started"], eager=True):
             proj[idx], done[idx] = pr, ev
             for t, (s1, s2, i, j, k) in enumerate(pl["targets"]):
                 if results[t] is not None:
                     continue
                 g = pl["base"] + t
-                slot = g % _hip.N_SLOTS
-                stream = _hip.slot_stream(slot, ipeps[s1]['A'].device)
+                home = g % _hip.N_SLOTS
+                dev = ipeps[s1]['A'].device
                 # the part that the first finished of the two pairs feeds on
-                # its own starts at once, the rest when the other pair is done
-                # too (same stream, same workspace)
+                # its own starts at once (on a stream whose pairs are past
+                # their eigensolvers, so that it holds up no projector
+                # products queued there later), the rest when the other pair
+                # is done too (same workspace)
                 if parts[t] is None and (proj[j] is not None or proj[i] is not None):
                     a = j if proj[j] is not None else i
+                    sl = home if clear(home) else job_slot[a] if clear(job_slot[a]) else home
+                    stream = _hip.slot_stream(sl, dev)
                     stream.wait_event(done[a])
                     with torch.cuda.stream(stream):
                         st = ipeps[s1]
-                        parts[t] = (_hip.RenormalizeParts(
+                        parts[t] = [_hip.RenormalizeParts(
                             (st['A'], st['C'], st['E']), k, ipeps.dims['chi'],
                             first_pair="j" if a == j else "i",
-                            slot=None if capture is not None else slot,
+                            slot=None if capture is not None else home,
                             precision=calc.precision,
-                            ws_key=None if capture is not None else g), a)
+                            ws_key=None if capture is not None else g), a, sl, None]
                         parts[t][0].first(flat(proj[a][0]), flat(proj[a][1]))
+                        parts[t][3] = torch.cuda.Event()
+                        parts[t][3].record(stream)
                 if parts[t] is not None and proj[i] is not None and proj[j] is not None:
                     b = i if parts[t][1] == j else j
+                    sl = home if clear(home) else parts[t][2]
+                    stream = _hip.slot_stream(sl, dev)
                     stream.wait_event(done[b])
+                    if sl != parts[t][2]:
+                        stream.wait_event(parts[t][3])   # the first part (same workspace)
                     with torch.cuda.stream(stream):
                         results[t] = parts[t][0].second(flat(proj[b][0]), flat(proj[b][1]))
         pl["parts"] = parts
@@ -281,26 +299,43 @@ def run_chain_pair(mover, ipeps, chain_a, chain_b, stagger=True):
         parts, results = [None] * len(targets), [None] * len(targets)
         flat = lambda P: P.reshape(-1, P.shape[-1])
         ends = []
-        for idx, pr, dn in calc.finish_each(ipeps, js):
+        job_slot = [slots[c][x % 2] for x in range(len(js))]
+
+        def clear(sl):
+            # every pair on slot sl has its projector products enqueued, so
+            # work added there now runs behind nothing but them
+            return all(proj[x] is not None for x in range(len(js)) if job_slot[x] == sl)
+
+        for idx, pr, dn in calc.finish_each(ipeps, js, eager=True):
             proj[idx], done[idx] = pr, dn
             for t, (s1, s2, i, j, kk) in enumerate(targets):
                 if results[t] is not None:
                     continue
-                slot = slots[c][t % 2]
-                stream = _hip.slot_stream(slot, dev)
+                home = slots[c][t % 2]
                 if parts[t] is None and (proj[j] is not None or proj[i] is not None):
                     a = j if proj[j] is not None else i
+                    # a part placed on a stream whose pair is still in its
+                    # eigensolver would hold up that pair's projector products,
+                    # queued behind it later: it goes to pair a's stream then
+                    sl = home if clear(home) else job_slot[a] if clear(job_slot[a]) else home
+                    stream = _hip.slot_stream(sl, dev)
                     stream.wait_event(done[a])
                     with torch.cuda.stream(stream):
                         st = ipeps[s1]
-                        parts[t] = (_hip.RenormalizeParts(
+                        parts[t] = [_hip.RenormalizeParts(
                             (st['A'], st['C'], st['E']), kk, ipeps.dims['chi'],
-                            first_pair="j" if a == j else "i", slot=slot,
-                            precision=calc.precision, ws_key=("chain", c, k, t)), a)
+                            first_pair="j" if a == j else "i", slot=sl,
+                            precision=calc.precision, ws_key=("chain", c, k, t)), a, sl, None]
                         parts[t][0].first(flat(proj[a][0]), flat(proj[a][1]))
+                        parts[t][3] = torch.cuda.Event()
+                        parts[t][3].record(stream)
                 if parts[t] is not None and proj[i] is not None and proj[j] is not None:
                     b = i if parts[t][1] == j else j
+                    sl = home if clear(home) else parts[t][2]
+                    stream = _hip.slot_stream(sl, dev)
                     stream.wait_event(done[b])
+                    if sl != parts[t][2]:
+                        stream.wait_event(parts[t][3])   # the first part (same workspace)
                     with torch.cuda.stream(stream):
                         results[t] = parts[t][0].second(flat(proj[b][0]), flat(proj[b][1]))
                         e = torch.cuda.Event()

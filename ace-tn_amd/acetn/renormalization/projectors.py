@@ -106,16 +106,19 @@ class ProjectorCalculator:
                                              mark=mark))
         return started
 
-    def finish_each(self, ipeps, started):
-        """Yields (index, (P1, P2), done_event) per started job, in order: the
-        host waits for that job's truncation count only, then enqueues its
+    def finish_each(self, ipeps, started, eager=False):
+        """Yields (index, (P1, P2), done_event) per started job: the host
+        waits for that job's truncation count only, then enqueues its
         projector products on the job's stream; ``done_event`` marks their
-        completion on that stream."""
+        completion on that stream.  Jobs come in index order, except that with ``eager``,
+        once a job's count is in, every later job whose count has arrived
+        too has its products enqueued before any of them is yielded (and is
+        yielded next), so that work the caller enqueues per yielded job is
+        never queued ahead of those products on their streams."""
         import torch
         D = ipeps.dims['bond']
-        for idx, job in enumerate(started):
-            if job.ready is not None:
-                job.ready.synchronize()
+
+        def finish(job):
             P1, P2 = job.finish()
             # the kept count: a device scalar for a captured pair (written on
             # every replay)
@@ -124,7 +127,22 @@ class ProjectorCalculator:
             done = torch.cuda.Event()
             done.record(job.stream)
             rows = P1.shape[0]
-            yield idx, (P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)), done
+            return (P1.view(rows // (D * D), D, D, -1), P2.view(rows // (D * D), D, D, -1)), done
+
+        pending = list(range(len(started)))
+        while pending:
+            idx = pending.pop(0)
+            if started[idx].ready is not None:
+                started[idx].ready.synchronize()
+            batch = [idx]
+            if eager:
+                for x in list(pending):
+                    if started[x].ready is None or started[x].ready.query():
+                        batch.append(x)
+                        pending.remove(x)
+            outs = [(x, finish(started[x])) for x in batch]
+            for x, (P, done) in outs:
+                yield x, P, done
         if started:
             self.last_spectrum = started[-1].spec
 

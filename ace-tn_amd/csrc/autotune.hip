@@ -148,9 +148,11 @@ int64_t autotune_load(const char* path) {
     const bool tile_ok = (wm == 1 && wn == 5) || (wm == 2 && wn == 5) || (wm == 2 && wn == 2) ||
                          (wm == 4 && wn == 4);
     // the wider FP64 variants exist for single-stride products: 2 x 5 tiles
-    // in variants 1-3, 1 x 5 tiles in variant 3
+    // in variants 1-3, 1 x 5 tiles in variant 3; two-piece split products
+    // have 64-deep k-tiles as variant 1
     const bool var_ok = var == 0 || (kind == 0 && wn == 5 && (flags & 1) &&
-                                     ((wm == 2 && var <= 3) || (wm == 1 && var == 3)));
+                                     ((wm == 2 && var <= 3) || (wm == 1 && var == 3))) ||
+                        (kind == 2 && var == 1);
     if (!tile_ok || !var_ok || ks < 1 || ks > 64) continue;
     TuneChoice c{wm, wn, ks, ms};
     c.var = var;

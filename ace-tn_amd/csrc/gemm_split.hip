@@ -171,21 +171,23 @@ __global__ __launch_bounds__(256) void split_transpose_kernel(const double* __re
 // the matrix edge are clamped to the last row (their results are never
 // stored), so the loads carry no predicates.  The next k-tile is in flight
 // in registers while the MFMAs run on the LDS tile (a second A tile in
-// flight measured no faster: the kernel is not load-latency bound).  Split-K
-// partials
+// flight measured no faster: the kernel is not load-latency bound).  BKT = 64
+// (two-piece products, tuner variant 1) stages two MFMA k-steps per barrier
+// pair: 44.0 against 47.5 us on the 5184 x 146 x 5184 range-finder product.
+// Split-K partials
 // are FP32, the accumulator type, so they lose nothing.  Offsets are 32-bit
 // (the host checks that every plane set stays below 2^31 elements).
-template <int WM, int WN, int SPLIT>
+template <int WM, int WN, int SPLIT, int BKT = BK>
 __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __bf16* __restrict__ Ap,
                                                         int64_t aplane,
                                                         const __bf16* __restrict__ Bp, int Kp,
                                                         double* __restrict__ C, int ksplit,
                                                         float* __restrict__ work) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
-  constexpr int LK = BK + KPAD;
-  constexpr int BOCT = SPLIT * BN * (BK / 8);  // 16-byte octets of B per k-tile
+  constexpr int LK = BKT + KPAD;
+  constexpr int BOCT = SPLIT * BN * (BKT / 8);  // 16-byte octets of B per k-tile
   constexpr int BV = (BOCT + NT - 1) / NT;
-  constexpr int AOCT = SPLIT * BM * (BK / 8);  // same for A
+  constexpr int AOCT = SPLIT * BM * (BKT / 8);  // same for A
   constexpr int AV = (AOCT + NT - 1) / NT;
   extern __shared__ __align__(16) char smem[];
   __bf16* As = reinterpret_cast<__bf16*>(smem);  // [SPLIT][BM][LK]
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __b
   const int kpart = blockIdx.z;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   int kchunk = (g.K + ksplit - 1) / ksplit;
-  kchunk = (kchunk + BK - 1) / BK * BK;
+  kchunk = (kchunk + BKT - 1) / BKT * BKT;
   const int kbeg = kpart * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
   const int64_t bplane = (int64_t)g.N * Kp;
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __b
 #pragma unroll
   for (int i = 0; i < AV; ++i) {
     const int o = min(tid + i * NT, AOCT - 1);
-    const int q = o % (BK / 8), ml = (o / (BK / 8)) % BM, s = o / (BK / 8 * BM);
+    const int q = o % (BKT / 8), ml = (o / (BKT / 8)) % BM, s = o / (BKT / 8 * BM);
     pa[i] = s * (int)aplane + min(m0 + ml, g.M - 1) * Kp + 8 * q;
     sa_off[i] = (s * BM + ml) * LK + 8 * q;
   }
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __b
 #pragma unroll
   for (int i = 0; i < BV; ++i) {
     const int o = min(tid + i * NT, BOCT - 1);
-    const int q = o % (BK / 8), nl = (o / (BK / 8)) % BN, s = o / (BK / 8 * BN);
+    const int q = o % (BKT / 8), nl = (o / (BKT / 8)) % BN, s = o / (BKT / 8 * BN);
     pb[i] = s * (int)bplane + min(n0 + nl, g.N - 1) * Kp + 8 * q;
     sb_off[i] = (s * BN + nl) * LK + 8 * q;
     b_k8[i] = 8 * q;
@@ -263,8 +265,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __b
   // smallest piece products first; the A fragments of every piece stay in
   // registers, B fragments are re-read from LDS per product (fewer live
   // registers, more resident waves)
-  auto compute = [&]() {
-    const int k0 = 8 * (lane >> 4);
+  auto compute = [&](int kk) {
+    const int k0 = kk + 8 * (lane >> 4);
     bf8 a[SPLIT][WM];
 #pragma unroll
     for (int s = 0; s < SPLIT; ++s)
@@ -295,15 +297,18 @@ __global__ __launch_bounds__(NT, 2) void gemm_split_kernel(GemmDesc g, const __b
     load_b(rb, kbeg);
     load_a(ra0, kbeg);
   }
-  for (int kt = kbeg; kt < kend; kt += BK) {
+  for (int kt = kbeg; kt < kend; kt += BKT) {
     __syncthreads();
     stage(ra0, rb, kt);
     __syncthreads();
-    if (kt + BK < kend) {
-      load_b(rb, kt + BK);
-      load_a(ra0, kt + BK);
+    if (kt + BKT < kend) {
+      load_b(rb, kt + BKT);
+      load_a(ra0, kt + BKT);
     }
-    compute();
+    // one MFMA k-step per 32 of the tile, in k order (the accumulation order
+    // of 32-deep tiles)
+#pragma unroll
+    for (int kk = 0; kk < BKT; kk += BK) compute(kk);
   }
 
   // one loop per destination, the choice made once (uniform per launch)
@@ -637,13 +642,13 @@ Tile choose_tile(const GemmDesc& g) {
 
 // Workgroups of one planar kernel instance resident per CU (registers and
 // LDS), from the runtime's occupancy calculator; 1 if it cannot say.
-template <int WM, int WN, int SPLIT>
+template <int WM, int WN, int SPLIT, int BKT = BK>
 int planar_wgs_per_cu() {
   static int n = 0;
   if (n == 0) {
     constexpr int BM = 32 * WM, BN = 32 * WN;
-    const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
-    const void* f = (const void*)gemm_split_kernel<WM, WN, SPLIT>;
+    const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BKT + KPAD);
+    const void* f = (const void*)gemm_split_kernel<WM, WN, SPLIT, BKT>;
     ACETN_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int v = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, f, NT, lds) != hipSuccess || v < 1) v = 1;
@@ -653,7 +658,15 @@ int planar_wgs_per_cu() {
 }
 
 template <int SPLIT>
-int planar_wgs_per_cu(const Tile& t) {
+int planar_wgs_per_cu(const Tile& t, int var = 0) {
+  if constexpr (SPLIT == 2) {
+    if (var == 1) {
+      if (t.wm == 2 && t.wn == 5) return planar_wgs_per_cu<2, 5, 2, 2 * BK>();
+      if (t.wm == 1 && t.wn == 5) return planar_wgs_per_cu<1, 5, 2, 2 * BK>();
+      if (t.wm == 4 && t.wn == 4) return planar_wgs_per_cu<4, 4, 2, 2 * BK>();
+      return planar_wgs_per_cu<2, 2, 2, 2 * BK>();
+    }
+  }
   if (t.wm == 2 && t.wn == 5) return planar_wgs_per_cu<2, 5, SPLIT>();
   if (t.wm == 1 && t.wn == 5) return planar_wgs_per_cu<1, 5, SPLIT>();
   if (t.wm == 4 && t.wn == 4) return planar_wgs_per_cu<4, 4, SPLIT>();
@@ -663,7 +676,7 @@ int planar_wgs_per_cu(const Tile& t) {
 // Split-K count: the largest that keeps tiles x parts within one round of
 // resident workgroups on the 256 CUs (a second, partly filled round costs as
 // much as a full one), each part at least 4 k-tiles long.  per_cu = 0 gives
-// the upper bound used to size the scratch (LDS limit only).
+// the upper bound used to size the scratch (LDS limit only, 32-deep tiles).
 int choose_ksplit_for(const GemmDesc& g, int split, int per_cu, const Tile& t) {
   const int BM = 32 * t.wm, BN = 32 * t.wn;
   const int64_t tiles = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
@@ -701,18 +714,20 @@ size_t plane_elems(const GemmDesc& g, int split) {
   return (bytes + 255) / 256 * 256 / sizeof(double);
 }
 
-template <int WM, int WN, int SPLIT>
+template <int WM, int WN, int SPLIT, int BKT = BK>
 void launch(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const __bf16* Bp, int Kp,
             double* C, int ks, double* work, hipStream_t s, SplitPlanes* Cp = nullptr) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
-  const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BK + KPAD);
-  planar_wgs_per_cu<WM, WN, SPLIT>();  // sets the dynamic-LDS attribute once
+  const size_t lds = sizeof(__bf16) * (size_t)SPLIT * (BM + BN) * (BKT + KPAD);
+  // the k loop reads whole tiles: the padded length must be a multiple
+  ACETN_REQUIRE(Kp % BKT == 0, "gemm_split: padded K is not a multiple of the k-tile");
+  planar_wgs_per_cu<WM, WN, SPLIT, BKT>();  // sets the dynamic-LDS attribute once
   dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, ks);
   ACETN_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "gemm_split: grid too large");
   float* wf = reinterpret_cast<float*>(work);
   hipEvent_t e0 = kernel_timing_begin(s);
-  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT>), grid, dim3(NT), lds, s, g, Ap, aplane, Bp,
-                     Kp, C, ks, wf);
+  hipLaunchKernelGGL((gemm_split_kernel<WM, WN, SPLIT, BKT>), grid, dim3(NT), lds, s, g, Ap, aplane,
+                     Bp, Kp, C, ks, wf);
   ACETN_HIP_CHECK(hipGetLastError());
   // algorithmic bytes: both operands as SPLIT bf16 planes, C once in FP64
   kernel_timing_end(e0, s, kKGemmSplit, 2.0 * g.M * g.N * (double)g.K,
@@ -757,6 +772,16 @@ void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, d
     __bf16* planes = reinterpret_cast<__bf16*>(work + partial_elems(g, c.ks));
     if (Bin != nullptr) planes = reinterpret_cast<__bf16*>(Bin->p);
     else split_into_planes<SPLIT>(B, g.bk, g.bn, g.K, g.N, Kp, planes, s);
+    if constexpr (SPLIT == 2) {
+      // variant 1: 64-deep k-tiles (two MFMA k-steps per barrier pair)
+      if (c.var == 1) {
+        if (c.wm == 2 && c.wn == 5) launch<2, 5, 2, 2 * BK>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+        else if (c.wm == 1 && c.wn == 5) launch<1, 5, 2, 2 * BK>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+        else if (c.wm == 4 && c.wn == 4) launch<4, 4, 2, 2 * BK>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+        else launch<2, 2, 2, 2 * BK>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
+        return;
+      }
+    }
     if (c.wm == 2 && c.wn == 5) launch<2, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
     else if (c.wm == 1 && c.wn == 5) launch<1, 5, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
     else if (c.wm == 4 && c.wn == 4) launch<4, 4, SPLIT>(g, Ap, aplane, planes, Kp, C, c.ks, work, s, Cp);
@@ -775,21 +800,31 @@ void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, d
     if (autotune_should_tune(key, g, s)) {
       std::vector<TuneChoice> cands;
       const Tile tiles[4] = {{1, 5}, {2, 5}, {2, 2}, {4, 4}};
-      for (const Tile& t : tiles) {
-        const int h = choose_ksplit_for(g, SPLIT, planar_wgs_per_cu<SPLIT>(t), t);
-        const int opts[4] = {1, h, 2 * h, h / 2};
-        for (int k : opts) {
-          if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < BK)) continue;
-          bool dup = false;
-          for (const TuneChoice& c : cands) dup |= c.wm == t.wm && c.wn == t.wn && c.ks == k;
-          if (!dup) cands.push_back(TuneChoice{t.wm, t.wn, k});
+      // two-piece products also try 64-deep k-tiles when the padded K allows
+      const int nvar = (SPLIT == 2 && Kp % (2 * BK) == 0) ? 2 : 1;
+      for (int var = 0; var < nvar; ++var)
+        for (const Tile& t : tiles) {
+          const int h = choose_ksplit_for(g, SPLIT, planar_wgs_per_cu<SPLIT>(t, var), t);
+          const int opts[4] = {1, h, 2 * h, h / 2};
+          for (int k : opts) {
+            if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < (var ? 2 * BK : BK))) continue;
+            bool dup = false;
+            for (const TuneChoice& c : cands)
+              dup |= c.wm == t.wm && c.wn == t.wn && c.ks == k && c.var == var;
+            if (!dup) {
+              TuneChoice c{t.wm, t.wn, k};
+              c.var = var;
+              cands.push_back(c);
+            }
+          }
         }
-      }
       autotune_run(key, cands, s, launch_choice);
       return;
     }
     TuneChoice tuned;
-    if (autotune_lookup(key, &tuned) && fits(tuned.ks)) ch = tuned;
+    if (autotune_lookup(key, &tuned) && fits(tuned.ks) &&
+        (tuned.var == 0 || (SPLIT == 2 && Kp % (2 * BK) == 0)))
+      ch = tuned;
   }
   launch_choice(ch);
 }

@@ -109,3 +109,33 @@ def test_wide_fp64_variants_are_exact(hip, tmp_path, wm, var):
         f.write(f"0 {M} {N} {K} 2 {wm} 5 1 0.0 {var}\n")   # flags 2: not single-stride
     hip.autotune_clear()
     assert hip.autotune_load(path) == 0
+
+
+@pytest.mark.parametrize("wm", [1, 2])
+def test_deep_k_tile_split_variant(hip, tmp_path, wm):
+    """The two-piece split product with 64-deep k-tiles (variant 1), forced
+    through a loaded table entry on a planar shape whose padded K is a
+    multiple of 64: with one k part it accumulates in the order of the
+    32-deep kernel, so the two agree bit for bit; with four parts the part
+    boundaries differ (288 against 320 columns), so they agree to the split
+    products' rounding, and both match numpy to that tolerance."""
+    rng = np.random.default_rng(wm)
+    M, N, K = 1040, 146, 1152
+    A, B = rng.standard_normal((M, K)), rng.standard_normal((K, N))
+    exact = A @ B
+    tol = 1e-4 * np.abs(exact).max()
+    for ks in (1, 4):
+        out = {}
+        for var in (0, 1):
+            path = os.path.join(tmp_path, f"s{ks}{var}.txt")
+            with open(path, "w") as f:
+                f.write(f"2 {M} {N} {K} 1 {wm} 5 {ks} 0.0 {var}\n")
+            hip.autotune_clear()
+            assert hip.autotune_load(path) == 1
+            hip.autotune("table")
+            out[var] = hip.gemm(dev(A), dev(B), pieces=2).cpu().numpy()
+            np.testing.assert_allclose(out[var], exact, rtol=0, atol=tol)
+        if ks == 1:
+            assert np.array_equal(out[0], out[1])
+        else:
+            np.testing.assert_allclose(out[1], out[0], rtol=0, atol=tol)

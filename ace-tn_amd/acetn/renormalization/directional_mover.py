@@ -269,8 +269,9 @@ def run_chain_pair(mover, ipeps, chain_a, chain_b, stagger=True):
         raise ValueError("run_chain_pair: chains of different length")
     chains = (chain_a, chain_b)
     slots = ((0, 1), (2, 3))
-    first_site = ipeps[ipeps.site_list[0]]['A']
-    dev = first_site.device
+    # the device of the first plaquette's sites (a network needs no site
+    # list here, only indexing by site)
+    dev = ipeps[tuple(mover._plan(ipeps, [chain_a[0]])[0][0][0][0])]['A'].device
     ev = torch.cuda.Event()
     ev.record(torch.cuda.current_stream(dev))
     for sl in range(1, _hip.N_SLOTS):

@@ -86,7 +86,11 @@ def physical_state(D, seed, steps=50, dtau=0.05):
     for s in ipeps.site_list:
         ipeps[s]['A'] = A[tuple(s)]
     ipeps.site_states_initialized = True
-    evolve(ipeps, dtau, steps, model_factory.create(ipeps.config.model), ipeps.config.evolution)
+    # evolve reports its progress on stdout (as the reference's does); the
+    # bench's stdout carries only its JSON line
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):
+        evolve(ipeps, dtau, steps, model_factory.create(ipeps.config.model), ipeps.config.evolution)
     out = {}
     for s in ipeps.site_list:
         a = ipeps[s]['A'].double().cpu()

@@ -107,14 +107,17 @@ class ProjectorCalculator:
         return started
 
     def finish_each(self, ipeps, started, eager=False):
-        """Yields (index, (P1, P2), done_event) per started job: the host
-        waits for that job's truncation count only, then enqueues its
-        projector products on the job's stream; ``done_event`` marks their
-        completion on that stream.  Jobs come in index order, except that with ``eager``,
-        once a job's count is in, every later job whose count has arrived
-        too has its products enqueued before any of them is yielded (and is
-        yielded next), so that work the caller enqueues per yielded job is
-        never queued ahead of those products on their streams."""
+        """Yields (index, (P1, P2), done_event) per started job, in index
+        order: the host waits for that job's truncation count only, then
+        enqueues its projector products on the job's stream; ``done_event``
+        marks their completion on that stream.  With ``eager``, once a job's
+        count is in, the jobs right after it whose counts have arrived too
+        have their products enqueued before any of them is yielded, so that
+        work the caller enqueues per yielded job is never queued ahead of
+        those products on their streams.  The order stays the index order
+        (callers pick the contraction order of a renormalisation from which
+        of its two pairs comes first, so a timing-dependent order would make
+        the results depend on timing at the rounding level)."""
         import torch
         D = ipeps.dims['bond']
 
@@ -135,11 +138,9 @@ class ProjectorCalculator:
             if started[idx].ready is not None:
                 started[idx].ready.synchronize()
             batch = [idx]
-            if eager:
-                for x in list(pending):
-                    if started[x].ready is None or started[x].ready.query():
-                        batch.append(x)
-                        pending.remove(x)
+            while eager and pending and (started[pending[0]].ready is None or
+                                         started[pending[0]].ready.query()):
+                batch.append(pending.pop(0))
             outs = [(x, finish(started[x])) for x in batch]
             for x, (P, done) in outs:
                 yield x, P, done

@@ -219,7 +219,11 @@ def run_ctmrg(precision, args, A, keep_state=False):
         n = _hip.autotune_load(args.tile_table)
         _hip.autotune("table")
         log(f"[{precision}] loaded {n} tile choices from {args.tile_table}")
-    elif mode != "off":
+    elif mode == "off":
+        # the built-in thresholds only (no table left from an earlier run)
+        _hip.autotune("off")
+        _hip.autotune_clear()
+    else:
         # one more iteration tunes the tile shape of each product shape (with
         # the projector pairs serialised, every product timed alone, or
         # concurrent as in the timed loop); the timed iterations then use the

@@ -789,14 +789,18 @@ void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, d
   };
   const Tile t0 = choose_tile(g);
   TuneChoice ch{t0.wm, t0.wn, ks};
-  // products that write C planes keep the threshold choice (their split-K
-  // reduction writes the planes, so the split count changes the kernels run)
-  if (Cp == nullptr && work_elems > 0) {
+  // products that write C planes reduce their split-K partials straight into
+  // the planes; a single part would leave the planes unwritten, so their
+  // candidates keep at least two parts (and the threshold choice when that
+  // is a single part)
+  if ((Cp == nullptr || ks > 1) && work_elems > 0) {
     auto fits = [&](int k) {
       return partial_elems(g, k) + (Bin ? 0 : plane_elems(g, SPLIT)) <= work_elems;
     };
     const TuneKey key{SPLIT, g.M, g.N, g.K,
-                      flags | (Bin ? 4 : 0) | (g.c_f32 ? 8 : 0) | (g.a_f32 ? 16 : 0)};
+                      flags | (Bin ? 4 : 0) | (g.c_f32 ? 8 : 0) | (g.a_f32 ? 16 : 0) |
+                          (Cp ? 32 : 0)};
+    const int kmin = Cp ? 2 : 1;
     if (autotune_should_tune(key, g, s)) {
       std::vector<TuneChoice> cands;
       const Tile tiles[4] = {{1, 5}, {2, 5}, {2, 2}, {4, 4}};
@@ -807,7 +811,8 @@ void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, d
           const int h = choose_ksplit_for(g, SPLIT, planar_wgs_per_cu<SPLIT>(t, var), t);
           const int opts[4] = {1, h, 2 * h, h / 2};
           for (int k : opts) {
-            if (k < 1 || k > 64 || !fits(k) || (k > 1 && g.K / k < (var ? 2 * BK : BK))) continue;
+            if (k < kmin || k > 64 || !fits(k) || (k > 1 && g.K / k < (var ? 2 * BK : BK)))
+              continue;
             bool dup = false;
             for (const TuneChoice& c : cands)
               dup |= c.wm == t.wm && c.wn == t.wn && c.ks == k && c.var == var;
@@ -822,7 +827,7 @@ void run(const GemmDesc& g, const __bf16* Ap, int64_t aplane, const double* B, d
       return;
     }
     TuneChoice tuned;
-    if (autotune_lookup(key, &tuned) && fits(tuned.ks) &&
+    if (autotune_lookup(key, &tuned) && fits(tuned.ks) && tuned.ks >= kmin &&
         (tuned.var == 0 || (SPLIT == 2 && Kp % (2 * BK) == 0)))
       ch = tuned;
   }

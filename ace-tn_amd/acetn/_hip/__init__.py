@@ -291,8 +291,12 @@ def slot_stream(slot, device):
 
 
 def make_site(A, C, E):
-    """Build an AcetnSite view of device tensors (made contiguous FP64); the
-    returned tuple keeps the tensors alive."""
+    """Build an AcetnSite view of contiguous FP64 device tensors; the returned
+    tuple keeps the tensors alive.  Other dtypes are refused (the kernels read
+    FP64)."""
+    for t in [A, *C, *E]:
+        if t.dtype != torch.float64:
+            raise TypeError(f"acetn HIP engine: site tensors must be float64, got {t.dtype}")
     keep = [A.contiguous()] + [c.contiguous() for c in C] + [e.contiguous() for e in E]
     s = AcetnSite()
     s.A = keep[0].data_ptr()

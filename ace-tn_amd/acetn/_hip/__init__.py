@@ -173,6 +173,11 @@ def _declare(lib):
         "acetn_hip_mlp_adam_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
                                                    ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                    ctypes.c_float, ctypes.c_float, c_vp]),
+        "acetn_hip_rsvd_product_workspace": (c_size, [ctypes.c_int, c_vp, c_i64, ctypes.c_int,
+                                                      ctypes.c_int]),
+        "acetn_hip_rsvd_product": (ctypes.c_int, [ctypes.c_int, c_vp, c_vp, c_i64, ctypes.c_int,
+                                                  ctypes.c_int, c_u64, c_u64, c_vp, c_vp, c_vp,
+                                                  c_vp, c_size, c_vp]),
         "acetn_hip_svd_workspace": (c_size, [c_i64, c_i64]),
         "acetn_hip_svd": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp,
                                          ctypes.POINTER(ctypes.c_int32), c_vp, c_size, c_vp]),
@@ -420,6 +425,40 @@ def eigh(G):
     ws = workspace(nb, G.device)
     check(lib().acetn_hip_eigh(ptr(G), q, ptr(w), ptr(V), ptr(ws), ws.numel(), stream_handle()))
     return w, V
+
+
+def rsvd_product(ops, q, niter, mid_qr, seed, stream_id):
+    """Randomized SVD of ops[0] @ ops[1] @ ... (device FP64 matrices) without
+    forming the product (acetn_hip_rsvd_product): (U, S, V) with q clamped to
+    min(rows, cols)."""
+    if not ops:
+        raise ValueError("rsvd_product: no factor")
+    for t in ops:
+        if t.dtype != torch.float64:
+            raise TypeError(f"rsvd_product: float64 matrices only, got {t.dtype}")
+        if t.dim() != 2:
+            raise ValueError("rsvd_product: factors must be matrices")
+    for a, b in zip(ops[:-1], ops[1:]):
+        if a.shape[1] != b.shape[0]:
+            raise ValueError(f"rsvd_product: inner dimensions {tuple(a.shape)} x {tuple(b.shape)}")
+    ops = [t.contiguous() for t in ops]
+    dims = [ops[0].shape[0]] + [t.shape[1] for t in ops]
+    m, n = dims[0], dims[-1]
+    q = min(int(q), m, n)
+    dev = ops[0].device
+    darr = (c_i64 * len(dims))(*dims)
+    parr = (c_vp * len(ops))(*[t.data_ptr() for t in ops])
+    U = torch.empty(m, q, dtype=torch.float64, device=dev)
+    S = torch.empty(q, dtype=torch.float64, device=dev)
+    V = torch.empty(n, q, dtype=torch.float64, device=dev)
+    nb = lib().acetn_hip_rsvd_product_workspace(len(ops), darr, q, int(niter), int(mid_qr))
+    if nb == 0:
+        check(1)
+    ws = workspace(nb, dev)
+    check(lib().acetn_hip_rsvd_product(len(ops), parr, darr, q, int(niter), int(mid_qr),
+                                       int(seed), int(stream_id), ptr(U), ptr(S), ptr(V), ptr(ws),
+                                       ws.numel(), stream_handle()))
+    return U, S, V
 
 
 def svd(M):

@@ -155,6 +155,78 @@ void small_svd(Ctx& c, const double* BT, int64_t n, int64_t q, double* w, double
 }
 
 // ---------------------------------------------------------------------------
+// The randomized SVD of a product M = M_0 M_1 ... M_{r-1} of dense row-major
+// FP64 matrices (M_i: dims[i] x dims[i+1]) without forming it: svd_lowrank
+// (r = 1, linalg/svd_lowrank.py:4-44), fused_matmul_svd_lowrank (r = 2,
+// linalg/fused_matmul_svd_lowrank.py:4-52) and fused_3matmul_svd_lowrank
+// (r = 4 with an orthonormalisation between the two halves of every power
+// iteration, mid_qr, linalg/fused_3matmul_svd_lowrank.py:4-56).  The sketch
+// is the Philox Gaussian of (seed, sstream); QR is CholeskyQR3 (the same
+// spans as Householder QR); the SVD of the projected q x n matrix is the
+// two-pass Gram + Jacobi of small_svd.  Outputs U (m x q), S (q, descending)
+// and V (n x q) with M ~ U diag(S) V^T; V = orthonormalised M^T U, whose
+// columns are V diag(S), so directions of numerically zero singular values
+// still come out orthonormal.
+// ---------------------------------------------------------------------------
+void rsvd_product(Ctx& c, int r, const double* const* ops, const int64_t* dims, int64_t q,
+                  int niter, bool mid_qr, uint64_t seed, uint64_t sstream, double* U, double* S,
+                  double* V) {
+  SplitScope sc(c, 0);
+  const int64_t m = dims[0], n = dims[r];
+  int64_t wmax = 0;
+  for (int i = 0; i <= r; ++i) wmax = std::max(wmax, dims[i]);
+  double* Om = c.take((size_t)n * q);
+  double* T0 = c.take((size_t)wmax * q);
+  double* T1 = c.take((size_t)wmax * q);
+  double* Y = c.take((size_t)m * q);
+  double* Z = c.take((size_t)n * q);
+  double* w = c.take((size_t)q);
+  double* UB = c.take((size_t)q * q);
+  int* cd = c.take_int(2);
+  // X (dims[r] x q) -> M X (dims[0] x q), right to left
+  auto apply = [&](const double* X, double* out) {
+    const double* cur = X;
+    for (int i = r - 1; i >= 0; --i) {
+      double* dst = (i == 0) ? out : (((r - 1 - i) & 1) ? T1 : T0);
+      mm(c, (int)dims[i], (int)q, (int)dims[i + 1], false, ops[i], dims[i + 1], false, cur, q, dst,
+         q);
+      cur = dst;
+    }
+  };
+  // X (dims[0] x q) -> M^T X (dims[r] x q), left to right
+  auto apply_t = [&](const double* X, double* out) {
+    const double* cur = X;
+    for (int i = 0; i < r; ++i) {
+      double* dst = (i == r - 1) ? out : ((i & 1) ? T1 : T0);
+      mm(c, (int)dims[i + 1], (int)q, (int)dims[i], true, ops[i], dims[i + 1], false, cur, q, dst,
+         q);
+      cur = dst;
+    }
+  };
+  if (c.launch) gaussian_sketch(Om, n, q, seed, sstream, c.st);
+  apply(Om, Y);
+  for (int it = 0; it < niter; ++it) {
+    const double* Yb = orthonormalize(c, Y, m, q);
+    apply_t(Yb, Z);
+    if (mid_qr) {
+      const double* Zb = orthonormalize(c, Z, n, q);
+      apply(Zb, Y);
+    } else {
+      apply(Z, Y);
+    }
+  }
+  const double* Qm = orthonormalize(c, Y, m, q);
+  apply_t(Qm, Z);                              // Z = M^T Q = B^T
+  small_svd(c, Z, n, q, w, UB);
+  if (c.launch) count_kept(w, (int)q, (int)q, 0.0, S, cd, c.st);  // S = sqrt(max(w, 0))
+  mm(c, (int)m, (int)q, (int)q, false, Qm, q, false, UB, q, U, q);
+  mm(c, (int)n, (int)q, (int)q, false, Z, q, false, UB, q, V, q);
+  const double* Vo = orthonormalize(c, V, n, q);
+  if (Vo != V && c.launch)
+    ACETN_HIP_CHECK(hipMemcpyAsync(V, Vo, sizeof(double) * n * q, hipMemcpyDeviceToDevice, c.st));
+}
+
+// ---------------------------------------------------------------------------
 // calculate_half_system, rsvd branch (projectors.py:138-174) with
 // fused_matmul_svd_lowrank (linalg/fused_matmul_svd_lowrank.py:4-52).
 // ---------------------------------------------------------------------------
@@ -959,6 +1031,42 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       c.mixed = precision == ACETN_HIP_MIXED;
       half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o);
+    });
+  });
+}
+
+static void check_rsvd_args(int nops, const int64_t* dims, int64_t q) {
+  ACETN_REQUIRE(nops >= 1 && nops <= 8 && dims != nullptr, "rsvd_product: 1 to 8 factors");
+  for (int i = 0; i <= nops; ++i)
+    ACETN_REQUIRE(dims[i] >= 1 && dims[i] < (int64_t(1) << 31), "rsvd_product: bad dimension");
+  ACETN_REQUIRE(q >= 1 && q <= std::min(dims[0], dims[nops]) && q <= 510,
+                "rsvd_product: q must be in [1, min(rows, cols)] and at most 510");
+}
+
+size_t acetn_hip_rsvd_product_workspace(int nops, const int64_t* dims, int64_t q, int niter,
+                                        int mid_qr) {
+  size_t r = 0;
+  guarded([&] {
+    check_rsvd_args(nops, dims, q);
+    ACETN_REQUIRE(niter >= 0 && niter <= 16, "rsvd_product: niter out of range");
+    const double* none[8] = {};
+    r = measure([&](Ctx& c) {
+      rsvd_product(c, nops, none, dims, q, niter, mid_qr != 0, 0, 0, nullptr, nullptr, nullptr);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_rsvd_product(int nops, const double* const* ops, const int64_t* dims, int64_t q,
+                           int niter, int mid_qr, uint64_t seed, uint64_t sketch_stream, double* U,
+                           double* S, double* V, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    check_rsvd_args(nops, dims, q);
+    ACETN_REQUIRE(ops != nullptr && U && S && V, "rsvd_product: null pointer");
+    for (int i = 0; i < nops; ++i) ACETN_REQUIRE(ops[i] != nullptr, "rsvd_product: null factor");
+    ACETN_REQUIRE(niter >= 0 && niter <= 16, "rsvd_product: niter out of range");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      rsvd_product(c, nops, ops, dims, q, niter, mid_qr != 0, seed, sketch_stream, U, S, V);
     });
   });
 }

@@ -132,6 +132,24 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
                               int64_t* host_cD_new, double* spectrum, void* workspace,
                               size_t ws_bytes, void* stream);
 
+/* The randomized SVD of a product M = M_0 M_1 ... M_{nops-1} of dense
+ * row-major FP64 device matrices without forming it.  Replaces the three
+ * functions of acetn/linalg: svd_lowrank (nops = 1, linalg/svd_lowrank.py:4-44),
+ * fused_matmul_svd_lowrank (nops = 2, linalg/fused_matmul_svd_lowrank.py:4-52)
+ * and fused_3matmul_svd_lowrank (nops = 4 with mid_qr = 1: an orthonormalisation
+ * between the two halves of each power iteration,
+ * linalg/fused_3matmul_svd_lowrank.py:4-56).  ops[i] is dims[i] x dims[i+1]
+ * (leading dimension dims[i+1]); q is the caller's q already clamped to
+ * min(dims[0], dims[nops]) (at most 510).  The sketch is the Philox Gaussian of
+ * (seed, sketch_stream) in place of torch.randn.  Outputs (device, caller
+ * allocated): U (dims[0] x q), S (q, descending), V (dims[nops] x q) with
+ * M ~ U diag(S) V^T, U and V with orthonormal columns. */
+size_t acetn_hip_rsvd_product_workspace(int nops, const int64_t* dims, int64_t q, int niter,
+                                        int mid_qr);
+int acetn_hip_rsvd_product(int nops, const double* const* ops, const int64_t* dims, int64_t q,
+                           int niter, int mid_qr, uint64_t seed, uint64_t sketch_stream, double* U,
+                           double* S, double* V, void* workspace, size_t ws_bytes, void* stream);
+
 /* The same computation split at the host readback of cD_new, so that the
  * independent projector pairs of a stage (the left/right or up/down move
  * pair of acetn/renormalization/ctmrg.py:26-31) can run concurrently on

@@ -1,7 +1,8 @@
 """Configuration dataclasses (acetn/ipeps/ipeps_config.py:9-109).
 
 Same keys and defaults as the reference, plus the engine's own:
-  ctmrg.backend     -- "hip" (the only engine here)
+  ctmrg.backend     -- "hip" (the only engine here; the reference's backend
+                       names "torch" and "cutensor" map to it with a note)
   ctmrg.precision   -- "fp64" (plain path) | "mixed" (split-bf16 matrix-core
                        contractions with FP32 accumulation, FP64 QR/SVD)
   ctmrg.sketch_seed -- seed of the Philox range-finder sketch (None: drawn
@@ -125,11 +126,15 @@ class IpepsConfig:
             self.device = torch.device("cuda", 0)
 
     def validate_backend(self):
-        """ipeps_config.py:103-109, re-targeted: 'cutensor' requests map to the
-        native 'hip' engine; 'torch' is not provided by this engine."""
+        """ipeps_config.py:103-109, re-targeted: the reference falls back from
+        'cutensor' to 'torch' with a note when the extension is missing; here
+        the engine's own 'hip' backend runs every request, so both of the
+        reference's backend names ('torch', its default, and 'cutensor') map
+        to 'hip' with a note, and other names raise."""
         for sec in (self.evolution, self.ctmrg):
-            if sec.backend == "cutensor":
-                tqdm.write("cuTENSOR backend requested; using the native backend='hip'")
+            if sec.backend in ("cutensor", "torch"):
+                tqdm.write(f"backend={sec.backend!r} requested; the MI355X engine runs it as "
+                           "backend='hip'")
                 sec.backend = "hip"
             if sec.backend != "hip":
                 raise ValueError(f"backend={sec.backend!r} is not provided by the MI355X engine "

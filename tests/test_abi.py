@@ -44,8 +44,11 @@ def test_config_validation():
                       evolution={'backend': 'cutensor'}, device="cpu")
     assert cfg.evolution.backend == "hip"
     assert cfg.device.type == "cuda"
+    # the reference's default backend name runs on the engine too
+    cfg = IpepsConfig(TN={'nx': 2, 'ny': 2}, evolution={'backend': 'torch'})
+    assert cfg.evolution.backend == "hip"
     with pytest.raises(ValueError):
-        IpepsConfig(TN={'nx': 2, 'ny': 2}, evolution={'backend': 'torch'})
+        IpepsConfig(TN={'nx': 2, 'ny': 2}, evolution={'backend': 'numpy'})
     with pytest.raises(TypeError):
         IpepsConfig(TN={'nx': 2, 'ny': 2}, ctmrg={'no_such_key': 1})
     with pytest.raises(NotImplementedError):

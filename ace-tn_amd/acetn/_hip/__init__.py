@@ -236,7 +236,13 @@ def stream_handle(device=None):
 
 
 def ptr(t):
-    return c_vp(t.data_ptr()) if t is not None else c_vp(0)
+    """Device address of a tensor for the C boundary.  Host tensors are
+    refused: a kernel would take a host address for device memory."""
+    if t is None:
+        return c_vp(0)
+    if t.device.type != "cuda":
+        raise ValueError(f"acetn HIP engine: tensor on {t.device}, expected a GPU tensor")
+    return c_vp(t.data_ptr())
 
 
 class Workspace:
@@ -299,9 +305,13 @@ def make_site(A, C, E):
     """Build an AcetnSite view of contiguous FP64 device tensors; the returned
     tuple keeps the tensors alive.  Other dtypes are refused (the kernels read
     FP64)."""
-    for t in [A, *C, *E]:
+    ts = [A, *C, *E]
+    for t in ts:
         if t.dtype != torch.float64:
             raise TypeError(f"acetn HIP engine: site tensors must be float64, got {t.dtype}")
+    for t in ts:
+        if t.device.type != "cuda":
+            raise ValueError(f"acetn HIP engine: site tensor on {t.device}, expected a GPU tensor")
     keep = [A.contiguous()] + [c.contiguous() for c in C] + [e.contiguous() for e in E]
     s = AcetnSite()
     s.A = keep[0].data_ptr()
@@ -442,6 +452,9 @@ def rsvd_product(ops, q, niter, mid_qr, seed, stream_id):
         if a.shape[1] != b.shape[0]:
             raise ValueError(f"rsvd_product: inner dimensions {tuple(a.shape)} x {tuple(b.shape)}")
     ops = [t.contiguous() for t in ops]
+    for t in ops:
+        if t.device.type != "cuda":
+            raise ValueError(f"rsvd_product: tensor on {t.device}, expected a GPU tensor")
     dims = [ops[0].shape[0]] + [t.shape[1] for t in ops]
     m, n = dims[0], dims[-1]
     q = min(int(q), m, n)

@@ -84,19 +84,23 @@ def test_learned_start_network_save_load_roundtrip(tmp_path):
         MLP.load(str(p), device="cpu")
 
 
-def test_site_struct_refuses_non_float64_tensors():
-    """The kernels read FP64: a float32 (or complex) site tensor is refused at
-    the boundary instead of being read as FP64 bytes (CPU tensors suffice to
-    check the guard, which runs before any device call)."""
+def test_boundary_refuses_host_and_non_float64_tensors():
+    """The kernels read FP64 device memory: a float32 (or complex) site
+    tensor is refused at the boundary instead of being read as FP64 bytes,
+    and a host tensor instead of being handed to a kernel as a device
+    address (both guards run before any device call)."""
     import torch
     from acetn import _hip
     A = torch.zeros(2, 2, 2, 2, 2, dtype=torch.float64)
     C = [torch.zeros(4, 4, dtype=torch.float64) for _ in range(4)]
     E = [torch.zeros(4, 4, 2, 2, dtype=torch.float64) for _ in range(4)]
-    s, keep = _hip.make_site(A, C, E)
-    assert s.D == 2 and s.d == 2 and s.C_dim[0][0] == 4
     for bad in (torch.float32, torch.complex128):
         with pytest.raises(TypeError, match="float64"):
             _hip.make_site(A.to(bad), C, E)
         with pytest.raises(TypeError, match="float64"):
             _hip.make_site(A, C[:3] + [C[3].to(bad)], E)
+    with pytest.raises(ValueError, match="GPU"):
+        _hip.make_site(A, C, E)
+    with pytest.raises(ValueError, match="GPU"):
+        _hip.ptr(A)
+    assert _hip.ptr(None).value is None

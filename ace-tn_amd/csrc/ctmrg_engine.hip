@@ -94,41 +94,100 @@ void quarter_tensor(Ctx& c, const acetn_site* s, int k, double* Q, bool f32 = fa
 // Orthonormal basis of range(Y), Y (m x q) row-major: shifted CholeskyQR3
 // (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020) with rank guard.
 // Result is written back into Y.
+//
+// A power-iterated sketch is graded far beyond what a Gram matrix resolves:
+// its columns carry the small singular directions only some 1e-7 below the
+// large ones or further.  The rank guard of the plain pass drops such
+// columns (pivot below 1e-14 of the largest); they are then re-derived from
+// the shifted pass's output: projected onto the complement of the kept
+// columns (G^+ from the kept columns' Gram matrix, so the kept ones need not
+// be orthonormal yet), factorised again, and added back.  Each such level
+// recovers some seven decades; what is still dropped after kRecoverLevels
+// of them (a sketch of a matrix of lower rank) is refilled at random.  Two
+// plain passes over the whole basis finish it.  Like
+// Householder QR, and unlike refilling every dropped column at random, this
+// keeps the directions the rounding-level content of those columns holds,
+// which after a power iteration are the next singular directions.  All of it
+// is skipped on the device when no column was dropped.
 // ---------------------------------------------------------------------------
 // Returns the buffer that holds the basis: Y, or, for a basis that only steers
 // the power iteration in mixed precision (steer), the output of the single
 // shifted CholeskyQR pass (well conditioned, not orthonormal to working
 // precision: the next orthonormalisation restores that).
+constexpr int kRecoverLevels = 3;
+
 const double* orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q, bool steer = false) {
   SplitScope sc(c, 0);
-  const int passes = (steer && c.mixed) ? 1 : 4;
+  const bool single = steer && c.mixed;
   double* G = c.take((size_t)q * q);
   double* Rinv = c.take((size_t)q * q);
   double* Yt = c.take((size_t)m * q);
   int* dead = c.take_int((size_t)q + 2);  // dead[0..q-1] flags, dead[q] = any dead
+  int* flags_a = single ? nullptr : c.take_int((size_t)q + 2);
+  int* flags_b = single ? nullptr : c.take_int((size_t)q + 2);
+  double* T = single ? nullptr : c.take((size_t)q * q);
+  double* T2 = single ? nullptr : c.take((size_t)q * q);
   const size_t pb = (size_t)q * (q + 1) / 2;
   double* scr = (pb * 8 > 150 * 1024) ? c.take(pb) : nullptr;
   const double u = 1.1102230246251565e-16;
-  // pass 0: shifted (Y -> Yt); pass 1: plain with rank guard (Yt -> Y), dead
-  // columns refilled; passes 2, 3 (Y -> Yt -> Y) orthogonalise the refilled
-  // columns and are skipped on the device when no column died.
-  for (int pass = 0; pass < passes; ++pass) {
-    double* src = (pass % 2 == 0) ? Y : Yt;
-    double* dst = (pass % 2 == 0) ? Yt : Y;
-    const int* run_if = (pass >= 2 && c.launch) ? dead + q : nullptr;
-    GemmDesc gg = dense_desc((int)q, (int)q, (int)m, true, q, false, q, q);
+  const int qi = (int)q, mi = (int)m;
+  const int* any = c.launch ? dead + q : nullptr;
+  // Gram matrix and inverse factor of src, then dst = src Rinv (or += with acc)
+  auto pass = [&](const double* src, double* dst, double shift, double rank_tol, int* dead_out,
+                  const int* only_if, const int* run_if, bool acc) {
+    GemmDesc gg = dense_desc(qi, qi, mi, true, q, false, q, q);
     gg.run_if = run_if;
     gemm(c, gg, src, src, G);
-    const double shift = (pass == 0) ? 11.0 * ((double)m * q + (double)q * (q + 1)) * u : 0.0;
-    if (c.launch) chol_rinv(G, Rinv, (int)q, shift, pass == 0 ? 0.0 : 1e-14, c.st, scr,
-                            pass == 1 ? dead : nullptr, pass >= 2 ? dead : nullptr);
-    GemmDesc gr = dense_desc((int)m, (int)q, (int)q, false, q, false, q, q);
+    if (c.launch) chol_rinv(G, Rinv, qi, shift, rank_tol, c.st, scr, dead_out, only_if);
+    GemmDesc gr = dense_desc(mi, qi, qi, false, q, false, q, q, 1.0, acc ? 1.0 : 0.0);
     gr.run_if = run_if;
     gemm(c, gr, src, Rinv, dst);
-    if (pass == 1 && c.launch) refill_dead_columns(dst, m, (int)q, dead, 0x243F6A8885A308D3ull, c.st);
+  };
+  // pass 0: shifted (Y -> Yt)
+  pass(Y, Yt, 11.0 * ((double)m * q + (double)q * (q + 1)) * u, 0.0, nullptr, nullptr, nullptr,
+       false);
+  if (single) return Yt;
+  // pass 1: plain with rank guard (Yt -> Y); dropped columns come out zero
+  pass(Yt, Y, 0.0, 1e-14, dead, nullptr, nullptr, false);
+  // recovery levels; ``prev`` flags the columns still dropped (level 0: the
+  // pass-1 ones), ``next`` receives those a level leaves dropped
+  int* prev = dead;
+  int* next = flags_a;
+  for (int level = 0; level < kRecoverLevels; ++level) {
+    const int* on = c.launch ? prev + q : nullptr;
+    // the dropped columns (of the pass-0 output, then of the previous
+    // level's residual), projected onto the complement of the columns Y
+    // holds: Yt <- (I - Y G^+ Y^T) Yt_dropped, G^+ = Rinv Rinv^T
+    if (c.launch) keep_dead_columns(Yt, m, qi, prev, c.st);
+    GemmDesc gg = dense_desc(qi, qi, mi, true, q, false, q, q);
+    gg.run_if = on;
+    gemm(c, gg, Y, Y, G);
+    if (c.launch) chol_rinv(G, Rinv, qi, 0.0, 1e-14, c.st, scr, nullptr, prev);
+    GemmDesc gt = dense_desc(qi, qi, mi, true, q, false, q, q);
+    gt.run_if = on;
+    gemm(c, gt, Y, Yt, T);                                   // Y^T Yt
+    GemmDesc g1 = dense_desc(qi, qi, qi, true, q, false, q, q);
+    g1.run_if = on;
+    gemm(c, g1, Rinv, T, T2);                                // Rinv^T (.)
+    GemmDesc g2 = dense_desc(qi, qi, qi, false, q, false, q, q);
+    g2.run_if = on;
+    gemm(c, g2, Rinv, T2, T);                                // Rinv (.)
+    GemmDesc g3 = dense_desc(mi, qi, qi, false, q, false, q, q, -1.0, 1.0);
+    g3.run_if = on;
+    gemm(c, g3, Y, T, Yt);                                   // Yt -= Y (.)
+    // factorise the projected columns and add the ones with rank back: the
+    // other columns of Yt are zero (their Rinv columns too), the dropped
+    // ones of Y are zero, so Y += Yt Rinv fills exactly the recovered ones
+    pass(Yt, Y, 0.0, 1e-14, next, prev, on, true);
+    if (c.launch) still_dead(next, prev, qi, c.st);
+    prev = next;
+    next = (next == flags_a) ? flags_b : flags_a;
   }
-  // with four passes the result is in Y whether or not passes 2, 3 ran
-  return passes == 1 ? Yt : Y;
+  if (c.launch) refill_dead_columns(Y, m, qi, prev, 0x243F6A8885A308D3ull, c.st);
+  // two plain passes over the whole basis (Y -> Yt -> Y)
+  pass(Y, Yt, 0.0, 1e-14, nullptr, dead, any, false);
+  pass(Yt, Y, 0.0, 1e-14, nullptr, dead, any, false);
+  return Y;
 }
 
 // ---------------------------------------------------------------------------

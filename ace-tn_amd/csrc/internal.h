@@ -238,6 +238,11 @@ void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double ra
 // row-major) by N(0,1) Philox vectors.
 void refill_dead_columns(double* Y, int64_t m, int q, const int* dead, uint64_t seed,
                          hipStream_t s);
+// Zero the columns of Y (m x q) not flagged in ``dead`` (device, q + 1 ints,
+// dead[q] = any); a no-op when none is flagged.
+void keep_dead_columns(double* Y, int64_t m, int q, const int* dead, hipStream_t s);
+// still[j] <- still[j] && dead[j], still[q] <- any (all zero when dead[q] is 0).
+void still_dead(int* still, const int* dead, int q, hipStream_t s);
 
 // Symmetric eigen-decomposition by cyclic parallel Jacobi.  G (q x q,
 // row-major) is overwritten; eigenvalues (descending) go to ``w`` and

@@ -455,6 +455,32 @@ __global__ void refill_kernel(double* __restrict__ Y, int64_t m, int q,
   }
 }
 
+// Zero the columns of Y (m x q) that ``dead`` does not flag; nothing at all
+// when dead[q] (any column flagged) is zero.
+__global__ void keep_dead_columns_kernel(double* __restrict__ Y, int64_t m, int q,
+                                         const int* __restrict__ dead) {
+  if (!dead[q]) return;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * q;
+       e += (int64_t)gridDim.x * blockDim.x)
+    if (!dead[e % q]) Y[e] = 0.0;
+}
+
+// still[j] <- still[j] && dead[j] (all zero when dead[q] is zero: ``still``
+// may then hold stale flags of a skipped factorisation); still[q] <- any.
+__global__ void still_dead_kernel(int* __restrict__ still, const int* __restrict__ dead, int q) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  const int on = dead[q];
+  for (int j = threadIdx.x; j < q; j += blockDim.x) {
+    const int v = (on && dead[j] && still[j]) ? 1 : 0;
+    still[j] = v;
+    if (v) any = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) still[q] = any;
+}
+
 // --------------------------------------------------------------------------
 // Jacobi eigensolver.  Round-robin ("circle method") parallel ordering over
 // n = q rounded up to even indices; the padding index, if any, never rotates.
@@ -1176,6 +1202,17 @@ void chol_rinv(const double* G, double* Rinv, int q, double shift_rel, double ra
   else
     hipLaunchKernelGGL(chol_rinv_kernel<false>, dim3(1), dim3(LA_NT), 0, s, G, Rinv, q, shift_rel,
                        rank_tol, gscratch, use_lds, dead_out, only_if_dead);
+  ACETN_HIP_CHECK(hipGetLastError());
+}
+
+void keep_dead_columns(double* Y, int64_t m, int q, const int* dead, hipStream_t s) {
+  const int blocks = (int)std::min<int64_t>((m * q + 255) / 256, 2048);
+  hipLaunchKernelGGL(keep_dead_columns_kernel, dim3(blocks), dim3(256), 0, s, Y, m, q, dead);
+  ACETN_HIP_CHECK(hipGetLastError());
+}
+
+void still_dead(int* still, const int* dead, int q, hipStream_t s) {
+  hipLaunchKernelGGL(still_dead_kernel, dim3(1), dim3(256), 0, s, still, dead, q);
   ACETN_HIP_CHECK(hipGetLastError());
 }
 

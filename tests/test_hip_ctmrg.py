@@ -119,6 +119,83 @@ def test_ctmrg_iterations_match_oracle(hip, D, chi):
     assert e_h == pytest.approx(e_o, rel=1e-11)
 
 
+def product_start(D, seed):
+    """The default start of an acetn network (site_tensor.py:120-136): the
+    polarised product state plus 1e-2 U[0,1) noise, normalised, with C/E
+    built from A.  Its half-system spectra fall over ten decades within the
+    first chi values, so the power-iterated sketch is graded far beyond what
+    its Gram matrix resolves."""
+    rng = np.random.default_rng(seed)
+    st = {}
+    for s in oc.site_list(2, 2):
+        a = 1e-2 * rng.random((D, D, D, D, 2))
+        a[0, 0, 0, 0, 0] += 1.0
+        st[s] = a / np.linalg.norm(a)
+    return oc.make_state(st)
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_half_projectors_on_graded_spectrum_match_oracle(hip, k):
+    """D=6 chi=36 from the default start: the kept count, all chi singular
+    values (down to ~1e-10 s0) and the projector columns as the oracle's
+    Householder-QR range finder gives them."""
+    D, chi = 6, 36
+    st = product_start(D, 31)
+    sites = oc.plaquette(2, 2, 0, 1, k)
+    seed, stream = 13, 2
+    ocalc = oc.ProjectorCalculator(omega_fn=SketchStream(seed, stream))
+    P1o, P2o = ocalc.calculate(st, sites, k, chi)
+    P1h, P2h, spec = hip.half_projectors(to_dev_site(st[sites[0]]), to_dev_site(st[sites[3]]), k,
+                                         chi, 2, 2, 1e-12, seed, stream)
+    s = spec.cpu().numpy()
+    so = ocalc.last_spectrum
+    rel = np.abs(s[:chi] / s[0] - so[:chi]) / so[:chi]
+    print(f"k={k}: kept hip {P1h.shape[1]} oracle {ocalc.last_cD}; s_chi/s_0 {so[chi - 1]:.3e}; "
+          f"max rel diff of the kept values {rel.max():.2e}")
+    assert P1h.shape[1] == ocalc.last_cD == chi
+    assert so[chi - 1] < 1e-8            # the case is graded
+    assert rel.max() < 1e-7
+    # every projector column as the oracle's, the leading ones (well-separated
+    # values) to near rounding
+    m1 = np.abs(hip.quarter_tensor(to_dev_site(st[sites[0]]), k).cpu().numpy()).max()
+    lead = int(np.sum(so[:chi] > 1e-5))
+    P1all = P1h.cpu().numpy() / m1
+    P1rall = P1o.reshape(-1, chi)
+    col = np.abs(P1all - P1rall).max(axis=0) / np.abs(P1rall).max(axis=0)
+    print(f"largest column difference of P1 {col.max():.1e} (relative to the column), "
+          f"{col[:lead].max():.1e} over the {lead} leading columns")
+    assert col[:lead].max() < 1e-10
+    assert col.max() < 1e-7
+
+
+def test_ctmrg_iterations_from_default_start_match_oracle(hip):
+    """Three CTMRG iterations at D=6 chi=36 from the default start (graded
+    spectra in every projector pair) against the oracle, same sketches.  For
+    scale: the oracle's own C/E move by ~1e-2 of their largest elements when
+    only the sketch changes, so agreement at 1e-7 needs the same range finder
+    down to the smallest kept values, not just an equally good one."""
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+    D, chi = 6, 36
+    st0 = product_start(D, 32)
+    seed = 515
+    ost = copy.deepcopy(st0)
+    omover = oc.DirectionalMover(oc.ProjectorCalculator(omega_fn=SketchStream(seed)), 2, 2, chi)
+    hst = DevState(st0, 2, 2, D, chi, seed)
+    hmover = DirectionalMover(Cfg(), hst.sketch)
+    for it in range(3):
+        oc.ctmrg_iteration(omover, ost)
+        ctmrg_iteration(hmover, hst)
+        h = hst.to_numpy()
+        err = max(np.abs(h[s][key][kk] - ost[s][key][kk]).max() / np.abs(ost[s][key][kk]).max()
+                  for s in h for key in ('C', 'E') for kk in range(4))
+        print(f"iteration {it}: max C/E difference {err:.1e} of the largest element")
+        assert_state_close(h, ost, rtol=1e-7)
+    e_o = om.measure(ost, 2, 2, omod.Heisenberg(1.0))['Energy']
+    e_h = om.measure(hst.to_numpy(), 2, 2, omod.Heisenberg(1.0))['Energy']
+    assert e_h == pytest.approx(e_o, rel=1e-12)
+
+
 @pytest.mark.parametrize("case", list(GOLDEN_CASES))
 def test_ctmrg_on_golden_fixture(hip, case):
     from acetn.renormalization.ctmrg import ctmrg_iteration

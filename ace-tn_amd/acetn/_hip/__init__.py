@@ -129,33 +129,40 @@ def _declare(lib):
         "acetn_hip_norm_tensor": (ctypes.c_int, [ctypes.POINTER(AcetnSite),
                                                  ctypes.POINTER(AcetnSite), ctypes.c_int, c_vp,
                                                  c_vp, c_vp, c_vp, c_size, c_vp]),
-        "acetn_hip_apply_gate_workspace": (c_size, [c_i64, c_i64]),
-        "acetn_hip_apply_gate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_size,
-                                                c_vp]),
-        "acetn_hip_positive_approx_workspace": (c_size, [c_i64, c_i64]),
-        "acetn_hip_positive_approx": (ctypes.c_int, [c_vp, c_i64, c_i64, c_dbl, c_vp,
+        "acetn_hip_apply_gate_workspace": (c_size, [c_i64, c_i64, c_i64]),
+        "acetn_hip_apply_gate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp,
+                                                c_size, c_vp]),
+        "acetn_hip_positive_approx_workspace": (c_size, [c_i64]),
+        "acetn_hip_positive_approx": (ctypes.c_int, [c_vp, c_i64, c_dbl, c_vp,
                                                      ctypes.POINTER(ctypes.c_int32), c_vp, c_size,
                                                      c_vp]),
         "acetn_hip_gauge_fix_workspace": (c_size, [c_i64, c_i64]),
         "acetn_hip_gauge_fix": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp,
                                                c_vp, c_vp, c_size, c_vp]),
-        "acetn_hip_als_workspace": (c_size, [c_i64, c_i64]),
-        "acetn_hip_als_initialize": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_size,
-                                                    c_vp]),
-        "acetn_hip_als_batch_workspace": (c_size, [c_i64, c_i64, c_i64]),
+        "acetn_hip_als_workspace": (c_size, [c_i64, c_i64, c_i64]),
+        "acetn_hip_als_initialize": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp,
+                                                    c_size, c_vp]),
+        "acetn_hip_als_batch_workspace": (c_size, [c_i64, c_i64, c_i64, c_i64]),
         "acetn_hip_als_solve_batch": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64,
-                                                     ctypes.c_int, c_dbl, ctypes.c_int, c_dbl,
-                                                     ctypes.c_int, c_vp, c_vp, c_vp, c_size,
-                                                     c_vp]),
-        "acetn_hip_als_solve": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, ctypes.c_int,
-                                               c_dbl, ctypes.c_int, c_dbl, ctypes.POINTER(c_i64),
-                                               ctypes.POINTER(c_dbl),
+                                                     c_i64, ctypes.c_int, c_dbl, ctypes.c_int,
+                                                     c_dbl, ctypes.c_int, c_vp, c_vp, c_vp,
+                                                     c_size, c_vp]),
+        "acetn_hip_als_solve": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64,
+                                               ctypes.c_int, c_dbl, ctypes.c_int, c_dbl,
+                                               ctypes.POINTER(c_i64), ctypes.POINTER(c_dbl),
                                                ctypes.POINTER(ctypes.c_int32), c_vp, c_size,
                                                c_vp]),
         "acetn_hip_als_solve_noalloc": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
-                                                       ctypes.c_int, c_dbl, ctypes.c_int, c_dbl,
+                                                       c_i64, ctypes.c_int, c_dbl, ctypes.c_int,
+                                                       c_dbl,
                                                        ctypes.POINTER(c_i64), ctypes.POINTER(c_dbl),
                                                        ctypes.POINTER(ctypes.c_int32), c_vp]),
+        "acetn_hip_finalize_reduced_workspace": (c_size, [c_i64, c_i64, c_i64]),
+        "acetn_hip_finalize_reduced": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64,
+                                                      c_vp, c_vp, c_vp, c_size, c_vp]),
+        "acetn_hip_recompose_workspace": (c_size, [c_i64, c_i64]),
+        "acetn_hip_recompose": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int,
+                                               c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
         "acetn_hip_finalize_bond_workspace": (c_size, [c_i64, c_i64]),
         "acetn_hip_finalize_bond": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                                    c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
@@ -223,6 +230,24 @@ def require_gpu():
     if not torch.cuda.is_available():
         raise RuntimeError("acetn HIP engine needs a visible AMD GPU (torch.cuda.is_available() "
                            "is False)")
+
+
+def to_engine(t, ipeps=None):
+    """``t`` as a contiguous FP64 GPU tensor: a GPU tensor stays on its
+    device, a host tensor goes to the network's device when ``ipeps`` is
+    given, else to the current one.  For the public functions that, like the
+    reference's, accept tensors on any device: the computation itself always
+    runs on the GPU."""
+    require_gpu()
+    if t.is_complex():
+        raise TypeError("acetn HIP engine: complex tensors are not supported (FP64 engine)")
+    if t.device.type == "cuda":
+        dev = t.device
+    elif ipeps is not None:
+        dev = ipeps.device
+    else:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return t.to(device=dev, dtype=torch.float64).contiguous()
 
 
 def check(status):
@@ -312,6 +337,14 @@ def make_site(A, C, E):
     for t in ts:
         if t.device.type != "cuda":
             raise ValueError(f"acetn HIP engine: site tensor on {t.device}, expected a GPU tensor")
+    if A.dim() != 5 or len(set(A.shape[:4])) != 1:
+        raise ValueError(f"acetn HIP engine: site tensor A has shape {tuple(A.shape)}, "
+                         "expected (D, D, D, D, d)")
+    D = A.shape[0]
+    if len(C) != 4 or len(E) != 4 or any(c.dim() != 2 for c in C) or \
+            any(e.dim() != 4 or tuple(e.shape[2:]) != (D, D) for e in E):
+        raise ValueError("acetn HIP engine: a site needs four corners (chi, chi) and four "
+                         "edges (chi, chi, D, D)")
     keep = [A.contiguous()] + [c.contiguous() for c in C] + [e.contiguous() for e in E]
     s = AcetnSite()
     s.A = keep[0].data_ptr()
@@ -406,13 +439,21 @@ def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None, pieces=0):
     """Dense GEMM on the matrix cores: alpha*op(A) op(B) + beta*C, FP64
     (pieces=0) or split-bf16 with FP32 accumulation (pieces=2: bf16x3,
     pieces=3: bf16x6); FP64 storage either way."""
+    if A.dim() != 2 or B.dim() != 2:
+        raise ValueError("gemm: A and B must be matrices")
     A = A.contiguous()
     B = B.contiguous()
     M = A.shape[1] if ta else A.shape[0]
     K = A.shape[0] if ta else A.shape[1]
     N = B.shape[0] if tb else B.shape[1]
+    _expect("B", B, (N, K) if tb else (K, N))
+    _expect("A", A, (K, M) if ta else (M, K))
     if C is None:
         C = torch.zeros(M, N, dtype=torch.float64, device=A.device)
+    else:
+        _expect("C", C, (M, N))
+        if not C.is_contiguous():
+            raise ValueError("gemm: C must be contiguous")
     nb = lib().acetn_hip_gemm_workspace(M, N, K)
     ws = workspace(nb, A.device)
     if pieces:
@@ -769,7 +810,11 @@ def _f64(*shape, device):
 
 def bond_decompose(A1, A2, k):
     """(a1q, a1r, a2q, a2r) of the bond frame k (tensor_update.py:52-68)."""
+    if A1.dim() != 5:
+        raise ValueError("acetn HIP engine: site tensors are (D, D, D, D, d)")
     D, d = A1.shape[0], A1.shape[4]
+    _expect("A1", A1, (D, D, D, D, d))
+    _expect("A2", A2, (D, D, D, D, d))
     nD = D * d
     dev = A1.device
     A1, A2 = A1.contiguous(), A2.contiguous()
@@ -785,7 +830,12 @@ def norm_tensor(site1, site2, k, a1q, a2q):
     """n12[y][x][Y][X] (full_update.py:163-227); site = (A, C, E)."""
     s1, keep1 = make_site(*site1)
     s2, keep2 = make_site(*site2)
-    nD = a1q.shape[3]
+    D, d = s1.D, s1.d
+    if (s2.D, s2.d) != (D, d):
+        raise ValueError("norm_tensor: the two sites' (D, d) differ")
+    nD = D * d
+    _expect("a1q", a1q, (D, D, D, nD))
+    _expect("a2q", a2q, (D, D, D, nD))
     dev = a1q.device
     n12 = _f64(nD, nD, nD, nD, device=dev)
     ws = workspace(lib().acetn_hip_norm_tensor_workspace(ctypes.byref(s1), ctypes.byref(s2), k), dev)
@@ -796,71 +846,99 @@ def norm_tensor(site1, site2, k, a1q, a2q):
     return n12
 
 
+def _expect(name, t, shape):
+    """Shape and dtype check of a bond-solver operand before its address
+    reaches a kernel (which reads exactly ``shape`` FP64 elements)."""
+    if t.dtype != torch.float64:
+        raise TypeError(f"acetn HIP engine: {name} must be float64, got {t.dtype}")
+    if tuple(t.shape) != tuple(int(x) for x in shape):
+        raise ValueError(f"acetn HIP engine: {name} has shape {tuple(t.shape)}, "
+                         f"expected {tuple(int(x) for x in shape)}")
+
+
 def apply_gate(a1r, a2r, gate):
+    """a12g (nD, nD, d, d) of full_update.py:77-86 from a1r, a2r (nD, D, d)."""
     nD, D, d = a1r.shape
+    _expect("a2r", a2r, (nD, D, d))
+    _expect("gate", gate, (d, d, d, d))
     a12g = _f64(nD, nD, d, d, device=a1r.device)
-    ws = workspace(lib().acetn_hip_apply_gate_workspace(D, d), a1r.device)
+    ws = workspace(lib().acetn_hip_apply_gate_workspace(nD, D, d), a1r.device)
     a1r, a2r, gate = a1r.contiguous(), a2r.contiguous(), gate.contiguous()
-    check(lib().acetn_hip_apply_gate(ptr(a1r), ptr(a2r),
-                                     ptr(gate), D, d, ptr(a12g), ptr(ws), ws.numel(),
-                                     stream_handle()))
+    check(lib().acetn_hip_apply_gate(ptr(a1r), ptr(a2r), ptr(gate), nD, D, d, ptr(a12g), ptr(ws),
+                                     ws.numel(), stream_handle()))
     return a12g
 
 
-def positive_approx(n12, D, d, cutoff):
-    """nz (nD, nD, nD^2) and the number of diagonal shifts applied."""
+def positive_approx(n12, cutoff):
+    """nz (nD, nD, nD^2) of full_update.py:262-297 and the number of diagonal
+    shifts applied."""
     nD = n12.shape[0]
+    _expect("n12", n12, (nD, nD, nD, nD))
     nz = _f64(nD, nD, nD * nD, device=n12.device)
     shifts = ctypes.c_int32(0)
-    ws = workspace(lib().acetn_hip_positive_approx_workspace(D, d), n12.device)
+    ws = workspace(lib().acetn_hip_positive_approx_workspace(nD), n12.device)
     n12 = n12.contiguous()
-    check(lib().acetn_hip_positive_approx(ptr(n12), D, d, cutoff, ptr(nz),
+    check(lib().acetn_hip_positive_approx(ptr(n12), nD, float(cutoff), ptr(nz),
                                           ctypes.byref(shifts), ptr(ws), ws.numel(),
                                           stream_handle()))
     return nz, shifts.value
 
 
-def gauge_fix(nz, a12g, D, d, atol):
+def gauge_fix(nz, a12g, atol):
     """(n12, a12g, nzxr_inv, nzyr_inv) of full_update.py:300-343."""
     nD = nz.shape[0]
+    _expect("nz", nz, (nD, nD, nD * nD))
+    d = a12g.shape[2] if a12g.dim() == 4 else 0
+    _expect("a12g", a12g, (nD, nD, d, d))
     dev = nz.device
     n12 = _f64(nD, nD, nD, nD, device=dev)
     a12g_out = _f64(nD, nD, d, d, device=dev)
     nzxr_inv, nzyr_inv = _f64(nD, nD, device=dev), _f64(nD, nD, device=dev)
-    ws = workspace(lib().acetn_hip_gauge_fix_workspace(D, d), dev)
+    ws = workspace(lib().acetn_hip_gauge_fix_workspace(nD, d), dev)
     nz, a12g = nz.contiguous(), a12g.contiguous()
-    check(lib().acetn_hip_gauge_fix(ptr(nz), ptr(a12g), D, d, atol,
+    check(lib().acetn_hip_gauge_fix(ptr(nz), ptr(a12g), nD, d, float(atol),
                                     ptr(n12), ptr(a12g_out), ptr(nzxr_inv), ptr(nzyr_inv), ptr(ws),
                                     ws.numel(), stream_handle()))
     return n12, a12g_out, nzxr_inv, nzyr_inv
 
 
 def als_initialize(a12g, D):
-    """The plain ALS start (als_solver.py:117-146): (a1r, a2r)."""
-    nD, _, d, _ = a12g.shape
+    """The plain ALS start (als_solver.py:117-146): (a1r, a2r), each (nD, D, d)."""
+    nD = a12g.shape[0]
+    d = a12g.shape[2] if a12g.dim() == 4 else 0
+    _expect("a12g", a12g, (nD, nD, d, d))
     dev = a12g.device
     a1r, a2r = _f64(nD, D, d, device=dev), _f64(nD, D, d, device=dev)
-    ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)
+    ws = workspace(lib().acetn_hip_als_workspace(nD, D, d), dev)
     a12g = a12g.contiguous()
-    check(lib().acetn_hip_als_initialize(ptr(a12g), D, d, ptr(a1r), ptr(a2r), ptr(ws),
+    check(lib().acetn_hip_als_initialize(ptr(a12g), nD, D, d, ptr(a1r), ptr(a2r), ptr(ws),
                                          ws.numel(), stream_handle()))
     return a1r, a2r
 
 
+def _als_operands(n12, a12g, a1r, a2r):
+    nD, D, d = a1r.shape
+    _expect("a1r", a1r, (nD, D, d))
+    _expect("a2r", a2r, (nD, D, d))
+    _expect("n12", n12, (nD, nD, nD, nD))
+    _expect("a12g", a12g, (nD, nD, d, d))
+    return nD, D, d
+
+
 def als_solve(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
     """Device ALS from the start (a1r, a2r); returns (a1r, a2r, sweeps, cost, flags)."""
-    nD, D, d = a1r.shape
+    nD, D, d = _als_operands(n12, a12g, a1r, a2r)
     dev = a1r.device
     if method not in ALS_METHODS:
         raise ValueError(f"Unknown ALS method: {method}")
     b1, b2 = a1r.clone().contiguous(), a2r.clone().contiguous()
     sweeps, cost, info = c_i64(0), c_dbl(0.0), ctypes.c_int32(0)
-    ws = workspace(lib().acetn_hip_als_workspace(D, d), dev)
+    ws = workspace(lib().acetn_hip_als_workspace(nD, D, d), dev)
     n12, a12g = n12.contiguous(), a12g.contiguous()
-    check(lib().acetn_hip_als_solve(ptr(n12), ptr(a12g), ptr(b1),
-                                    ptr(b2), D, d, int(niter), float(tol), ALS_METHODS[method],
-                                    float(epsilon), ctypes.byref(sweeps), ctypes.byref(cost),
-                                    ctypes.byref(info), ptr(ws), ws.numel(), stream_handle()))
+    check(lib().acetn_hip_als_solve(ptr(n12), ptr(a12g), ptr(b1), ptr(b2), nD, D, d, int(niter),
+                                    float(tol), ALS_METHODS[method], float(epsilon),
+                                    ctypes.byref(sweeps), ctypes.byref(cost), ctypes.byref(info),
+                                    ptr(ws), ws.numel(), stream_handle()))
     return b1, b2, sweeps.value, cost.value, info.value
 
 
@@ -872,10 +950,11 @@ def als_solve_batch(n12, a12g, D, niter, tol, method, epsilon, starts=None):
     synchronisation."""
     if method not in ALS_METHODS:
         raise ValueError(f"Unknown ALS method: {method}")
-    count, nD = int(a12g.shape[0]), int(a12g.shape[1])
-    d = int(a12g.shape[3])
-    if tuple(n12.shape) != (count, nD, nD, nD, nD) or tuple(a12g.shape) != (count, nD, nD, d, d):
-        raise ValueError("als_solve_batch: n12 / a12g shapes do not match")
+    if a12g.dim() != 5:
+        raise ValueError("als_solve_batch: a12g must be (count, nD, nD, d, d)")
+    count, nD, d = int(a12g.shape[0]), int(a12g.shape[1]), int(a12g.shape[3])
+    _expect("a12g", a12g, (count, nD, nD, d, d))
+    _expect("n12", n12, (count, nD, nD, nD, nD))
     dev = a12g.device
     n12, a12g = n12.contiguous(), a12g.contiguous()
     if starts is None:
@@ -883,12 +962,12 @@ def als_solve_batch(n12, a12g, D, niter, tol, method, epsilon, starts=None):
         a2 = torch.empty_like(a1)
     else:
         a1, a2 = (t.clone().contiguous() for t in starts)
-        if tuple(a1.shape) != (count, nD, D, d) or tuple(a2.shape) != (count, nD, D, d):
-            raise ValueError("als_solve_batch: start shapes do not match")
+        _expect("a1r", a1, (count, nD, D, d))
+        _expect("a2r", a2, (count, nD, D, d))
     cost = torch.empty(count, dtype=torch.float64, device=dev)
     info = torch.empty(count, 2, dtype=torch.int32, device=dev)
-    ws = workspace(lib().acetn_hip_als_batch_workspace(count, D, d), dev)
-    check(lib().acetn_hip_als_solve_batch(ptr(n12), ptr(a12g), ptr(a1), ptr(a2), count, D, d,
+    ws = workspace(lib().acetn_hip_als_batch_workspace(count, nD, D, d), dev)
+    check(lib().acetn_hip_als_solve_batch(ptr(n12), ptr(a12g), ptr(a1), ptr(a2), count, nD, D, d,
                                           int(niter), float(tol), ALS_METHODS[method],
                                           float(epsilon), 1 if starts is None else 0, ptr(cost),
                                           ptr(info), ptr(ws), ws.numel(), stream_handle()))
@@ -899,14 +978,14 @@ def als_solve_noalloc(n12, a12g, a1r, a2r, niter, tol, method, epsilon):
     """als_solve through the entry point that allocates nothing (its scratch
     is a fixed device array of the library, nD * D <= 128); a1r/a2r are
     updated in place.  Returns (sweeps, cost, flags)."""
-    nD, D, d = a1r.shape
+    nD, D, d = _als_operands(n12, a12g, a1r, a2r)
     if method not in ALS_METHODS:
         raise ValueError(f"Unknown ALS method: {method}")
     for t in (n12, a12g, a1r, a2r):
         if not t.is_contiguous():
             raise ValueError("als_solve_noalloc: tensors must be contiguous")
     sweeps, cost, info = c_i64(0), c_dbl(0.0), ctypes.c_int32(0)
-    check(lib().acetn_hip_als_solve_noalloc(ptr(n12), ptr(a12g), ptr(a1r), ptr(a2r), D, d,
+    check(lib().acetn_hip_als_solve_noalloc(ptr(n12), ptr(a12g), ptr(a1r), ptr(a2r), nD, D, d,
                                             int(niter), float(tol), ALS_METHODS[method],
                                             float(epsilon), ctypes.byref(sweeps),
                                             ctypes.byref(cost), ctypes.byref(info),
@@ -918,6 +997,14 @@ def finalize_bond(a1r, a2r, nzyr_inv, nzxr_inv, a1q, a2q, k):
     """Updated site tensors (site frame, unit norm) of full_update.py:123-161
     + tensor_update.py:26-38,70-75."""
     nD, D, d = a1r.shape
+    if nD != D * d:
+        raise ValueError(f"finalize_bond: reduced dimension {nD} != D d = {D * d}")
+    _expect("a2r", a2r, (nD, D, d))
+    _expect("a1q", a1q, (D, D, D, nD))
+    _expect("a2q", a2q, (D, D, D, nD))
+    for name, g in (("nzyr_inv", nzyr_inv), ("nzxr_inv", nzxr_inv)):
+        if g is not None:
+            _expect(name, g, (nD, nD))
     dev = a1r.device
     A1, A2 = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
     ws = workspace(lib().acetn_hip_finalize_bond_workspace(D, d), dev)
@@ -932,10 +1019,60 @@ def finalize_bond(a1r, a2r, nzyr_inv, nzxr_inv, a1q, a2q, k):
     return A1, A2
 
 
+def finalize_reduced(a1r, a2r, nzyr_inv, nzxr_inv):
+    """finalize_reduced_tensors (full_update.py:123-161): the rebalanced
+    (a1r, a2r), each (nD, D, d); the gauge inverses (nD, nD) or None."""
+    nD, D, d = a1r.shape
+    _expect("a2r", a2r, (nD, D, d))
+    if (nzyr_inv is None) != (nzxr_inv is None):
+        raise ValueError("finalize_reduced: give both gauge inverses or neither")
+    for name, g in (("nzyr_inv", nzyr_inv), ("nzxr_inv", nzxr_inv)):
+        if g is not None:
+            _expect(name, g, (nD, nD))
+    dev = a1r.device
+    b1, b2 = _f64(nD, D, d, device=dev), _f64(nD, D, d, device=dev)
+    ws = workspace(lib().acetn_hip_finalize_reduced_workspace(nD, D, d), dev)
+    a1r, a2r = a1r.contiguous(), a2r.contiguous()
+    gy = nzyr_inv.contiguous() if nzyr_inv is not None else None
+    gx = nzxr_inv.contiguous() if nzxr_inv is not None else None
+    check(lib().acetn_hip_finalize_reduced(ptr(a1r), ptr(a2r), ptr(gy), ptr(gx), nD, D, d,
+                                           ptr(b1), ptr(b2), ptr(ws), ws.numel(),
+                                           stream_handle()))
+    return b1, b2
+
+
+def recompose(a1r, a2r, a1q, a2q, k=0, normalize=False):
+    """recompose_site_tensors (tensor_update.py:70-75): (a1, a2), each
+    (D, D, D, D, d), rotated from the bond frame by k (k = 0: the bond frame),
+    each divided by its norm when ``normalize``."""
+    nD, D, d = a1r.shape
+    if nD != D * d:
+        raise ValueError(f"recompose: reduced dimension {nD} != D d = {D * d}")
+    _expect("a2r", a2r, (nD, D, d))
+    _expect("a1q", a1q, (D, D, D, nD))
+    _expect("a2q", a2q, (D, D, D, nD))
+    dev = a1r.device
+    A1, A2 = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
+    ws = workspace(lib().acetn_hip_recompose_workspace(D, d), dev)
+    a1r, a2r, a1q, a2q = (t.contiguous() for t in (a1r, a2r, a1q, a2q))
+    check(lib().acetn_hip_recompose(ptr(a1r), ptr(a2r), ptr(a1q), ptr(a2q), int(k) & 3,
+                                    1 if normalize else 0, D, d, ptr(A1), ptr(A2), ptr(ws),
+                                    ws.numel(), stream_handle()))
+    return A1, A2
+
+
 def simple_update_bond(A1, A2, k, lam1, lam2, gate, cutoff=1e-12):
     """(A1', A2', lambda') of simple_update.py:28-76; lam1/lam2 are four
     device D-vectors (None for the bond's own leg)."""
+    if A1.dim() != 5:
+        raise ValueError("acetn HIP engine: site tensors are (D, D, D, D, d)")
     D, d = A1.shape[0], A1.shape[4]
+    _expect("A1", A1, (D, D, D, D, d))
+    _expect("A2", A2, (D, D, D, D, d))
+    _expect("gate", gate, (d, d, d, d))
+    for lam in (*lam1, *lam2):
+        if lam is not None:
+            _expect("lambda", lam, (D,))
     dev = A1.device
     A1o, A2o = _f64(D, D, D, D, d, device=dev), _f64(D, D, D, D, d, device=dev)
     lam = _f64(D, device=dev)

@@ -1,12 +1,14 @@
 """Site tensor with its corner and edge environment
-(acetn/ipeps/site_tensor.py:4-202).  All tensors live on the GPU in float64.
-A is (l,u,r,d,p); C[k] is (chi,chi); E[k] is (chi,chi,D_ket,D_bra)."""
+(acetn/ipeps/site_tensor.py:4-202).  A is (l,u,r,d,p); C[k] is (chi,chi);
+E[k] is (chi,chi,D_ket,D_bra).  A stand-alone SiteTensor lives on the host
+by default, as the reference's does; the networks of the engine put theirs
+on the GPU in float64."""
 import torch
 
 
 class SiteTensor:
     def __init__(self, dims, site_tensor=None, site_state=(1., 0.), dtype=torch.float64,
-                 device=torch.device("cuda", 0)):
+                 device=torch.device("cpu")):
         self.dims = dims
         self.dtype = dtype
         self.device = device

@@ -1,5 +1,6 @@
 """Reduced density matrices (acetn/measurement/rdm.py:6-154) on the HIP
-engine: acetn_hip_site_rdm / acetn_hip_bond_rdm."""
+engine: acetn_hip_site_rdm / acetn_hip_bond_rdm.  Sites held on the host
+are computed on the GPU and their RDMs returned on the host."""
 from .. import _hip
 from ..ipeps.bond import Bond
 
@@ -16,11 +17,18 @@ class RDM:
 
     def _site(self, s):
         st = self.ipeps[s]
-        return st['A'], st['C'], st['E']
+        eng = _hip.to_engine
+        return eng(st['A']), [eng(c) for c in st['C']], [eng(e) for e in st['E']]
 
     def build_site_rdm(self, site):
-        return _hip.site_rdm(self._site(tuple(site)))
+        """rdm.py:35-67."""
+        site = tuple(site)
+        home = self.ipeps[site]['A']
+        return _hip.site_rdm(self._site(site)).to(device=home.device, dtype=home.dtype)
 
     def build_bond_rdm(self, bond):
+        """rdm.py:69-99."""
         s1, s2, k = bond
-        return _hip.bond_rdm(self._site(s1), self._site(s2), k)
+        home = self.ipeps[s1]['A']
+        rho = _hip.bond_rdm(self._site(s1), self._site(s2), k)
+        return rho.to(device=home.device, dtype=home.dtype)

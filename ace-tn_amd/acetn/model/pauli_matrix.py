@@ -28,6 +28,14 @@ class PauliMatrix:
             return PauliMatrix(new_key, self.dtype, self.device, mat=phase * torch.kron(self.mat, other.mat))
         return PauliMatrix(self.key, self.dtype, self.device, mat=other * self.mat)
 
+    def __imul__(self, other):
+        """``m *= other`` (pauli_matrix.py:74-90): with a PauliMatrix the
+        plain Kronecker product of the two matrices as a bare tensor (no
+        real-Y sign, unlike ``*``); with a scalar, the scaled PauliMatrix."""
+        if isinstance(other, PauliMatrix):
+            return torch.kron(self.mat, other.mat)
+        return PauliMatrix(self.key, self.dtype, self.device, mat=other * self.mat)
+
     def __rmul__(self, other):
         return PauliMatrix(self.key, self.dtype, self.device, mat=other * self.mat)
 

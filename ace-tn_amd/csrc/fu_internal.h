@@ -51,6 +51,8 @@ void su_split(const double* U, const double* S, const double* V, int nD, int bD,
 // (1: a Cholesky factorisation failed and LU was used; 2: singular system).
 size_t als_work_elems(int nD, int D, int d);
 size_t als_lds_bytes(int nD, int D, int d);
+// Raises unless the one-workgroup ALS holds the problem (method 0 cholesky, 1 pinv).
+void als_check_shape(int nD, int D, int d, int method);
 // With count > 1, problem b's n12, a12g, a1r, a2r, gwork, out_cost and
 // out_info follow problem b-1's contiguously (one workgroup per problem).
 void als_solve_device(const double* n12, const double* a12g, double* a1r, double* a2r, int nD,

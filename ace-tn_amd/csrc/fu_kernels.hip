@@ -912,14 +912,19 @@ size_t als_work_elems(int nD, int D, int d) {
   return nR * nR + nR * d;
 }
 
+void als_check_shape(int nD, int D, int d, int method) {
+  ACETN_REQUIRE(method == 0 || method == 1, "als: unknown method");
+  ACETN_REQUIRE(method == 0 || nD * D <= 256, "als: pinv needs nD * D <= 256");
+  ACETN_REQUIRE(als_lds_bytes(nD, D, d) <= 150 * 1024,
+                "als: reduced tensors too large for one workgroup");
+}
+
 void als_solve_device(const double* n12, const double* a12g, double* a1r, double* a2r, int nD,
                       int D, int d, int niter, double tol, double eps, double* gwork,
                       double* out_cost, int* out_info, hipStream_t s, int method, int count) {
-  ACETN_REQUIRE(method == 0 || method == 1, "als: unknown method");
   ACETN_REQUIRE(count >= 1, "als: empty batch");
-  ACETN_REQUIRE(method == 0 || nD * D <= 256, "als: pinv needs nD * D <= 256");
+  als_check_shape(nD, D, d, method);
   const size_t lds = als_lds_bytes(nD, D, d);
-  ACETN_REQUIRE(lds <= 150 * 1024, "als: reduced tensors too large for one workgroup");
   static bool attr = false;
   if (!attr) {
     ACETN_HIP_CHECK(hipFuncSetAttribute((const void*)als_kernel,

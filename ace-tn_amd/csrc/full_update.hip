@@ -21,6 +21,16 @@ FuDims fu_dims(int64_t D, int64_t d) {
   return FuDims{D, d, D * d};
 }
 
+// Dimensions of the bond solver (gate, positive_approx, gauge_fix, ALS):
+// reduced tensors (nD, D, d) with any reduced dimension nD, as ALSSolver
+// takes ar_shape = (nD, bD, pD) (als_solver.py:10-32); a full-update step has
+// nD = min(D^3, D d).
+FuDims solver_dims(int64_t nD, int64_t D, int64_t d) {
+  ACETN_REQUIRE(nD >= 1 && nD <= kQrMaxN && D >= 1 && D <= 64 && d >= 1 && d <= 8,
+                "bond solver: unsupported reduced tensor shape (nD, D, d)");
+  return FuDims{D, d, nD};
+}
+
 // site-frame strides of A (l,u,r,d,p) contiguous
 void site_strides(int64_t D, int64_t d, int64_t st[4]) {
   st[0] = D * D * D * d; st[1] = D * D * d; st[2] = D * d; st[3] = d;
@@ -104,6 +114,7 @@ void apply_gate(Ctx& c, const double* a1r, const double* a2r, const double* gate
 void positive_approx(Ctx& c, const double* n12, const FuDims& f, double cutoff, double* nz,
                      int* host_shifts) {
   const int q = (int)(f.nD * f.nD);
+  ACETN_REQUIRE(q <= 510, "positive_approx: nD^2 must be at most 510");
   double* N = c.take((size_t)q * q);
   double* G = c.take((size_t)q * q);
   double* w = c.take((size_t)q);
@@ -233,6 +244,8 @@ __global__ void plain_start_batch_kernel(const double* __restrict__ U,
 
 void plain_start(Ctx& c, const double* a12g, const FuDims& f, double* a1r, double* a2r) {
   const int64_t D = f.D, d = f.d, nD = f.nD, n = nD * d;
+  ACETN_REQUIRE(n <= kSvdMaxN && D <= n,
+                "ALS plain start: needs nD d <= 32 and D <= nD d (the SVD of a12g as (nD d, nD d))");
   double* M = c.take((size_t)n * n);
   double* U = c.take((size_t)n * n);
   double* sv = c.take((size_t)n);
@@ -286,15 +299,15 @@ void recompose(Ctx& c, const double* b1, const double* b2, const double* a1q, co
 }
 
 // ---------------------------------------------------------------------------
-// finalize_reduced_tensors (full_update.py:123-161) + recompose_site_tensors
-// (tensor_update.py:70-75) + normalisation + the inverse bond permutation
-// (tensor_update.py:26-38).  A1_out/A2_out are in the site frame.
+// finalize_reduced_tensors (full_update.py:123-161): the gauge inverses (when
+// given), then the QR-QR-SVD rebalancing with sqrt(s / |s|), then the
+// bond-index sign gauge; b1/b2 (nD, D, d) receive the result.
 // ---------------------------------------------------------------------------
-void finalize_bond(Ctx& c, const double* a1r_in, const double* a2r_in, const double* nzyr_inv,
-                   const double* nzxr_inv, const double* a1q, const double* a2q, int k,
-                   const FuDims& f, double* A1_out, double* A2_out) {
-  k &= 3;
+void finalize_reduced(Ctx& c, const double* a1r_in, const double* a2r_in, const double* nzyr_inv,
+                      const double* nzxr_inv, const FuDims& f, double* b1, double* b2) {
   const int64_t D = f.D, d = f.d, nD = f.nD, Dd = D * d;
+  ACETN_REQUIRE(D <= kSvdMaxN && D <= nD * d,
+                "finalize_reduced: needs D <= 32 and D <= nD d (QR of (nD d, D))");
   double* a1r = c.take((size_t)nD * Dd);
   double* a2r = c.take((size_t)nD * Dd);
   double* P = c.take((size_t)nD * Dd);
@@ -309,8 +322,6 @@ void finalize_bond(Ctx& c, const double* a1r_in, const double* a2r_in, const dou
   double* V = c.take((size_t)D * D);
   double* r1n = c.take((size_t)D * D);
   double* r2n = c.take((size_t)D * D);
-  double* b1 = c.take((size_t)nD * Dd);
-  double* b2 = c.take((size_t)nD * Dd);
   if (nzyr_inv != nullptr || !c.run) {
     mm(c, (int)nD, (int)Dd, (int)nD, false, nzyr_inv, nD, false, a1r_in, Dd, a1r, Dd);
     mm(c, (int)nD, (int)Dd, (int)nD, false, nzxr_inv, nD, false, a2r_in, Dd, a2r, Dd);
@@ -342,7 +353,21 @@ void finalize_bond(Ctx& c, const double* a1r_in, const double* a2r_in, const dou
                 grouped((int)d, 1, Dd), plain(d)),
        q2, r2n, b2);
   if (c.launch) bond_gauge(b1, b2, (int)nD, (int)nD, (int)D, (int)d, c.st);
-  recompose(c, b1, b2, a1q, a2q, k, f, true, A1_out, A2_out);
+}
+
+// ---------------------------------------------------------------------------
+// finalize_reduced + recompose_site_tensors (tensor_update.py:70-75) +
+// normalisation + the inverse bond permutation (tensor_update.py:26-38).
+// A1_out/A2_out are in the site frame.
+// ---------------------------------------------------------------------------
+void finalize_bond(Ctx& c, const double* a1r_in, const double* a2r_in, const double* nzyr_inv,
+                   const double* nzxr_inv, const double* a1q, const double* a2q, int k,
+                   const FuDims& f, double* A1_out, double* A2_out) {
+  const int64_t Dd = f.D * f.d;
+  double* b1 = c.take((size_t)f.nD * Dd);
+  double* b2 = c.take((size_t)f.nD * Dd);
+  finalize_reduced(c, a1r_in, a2r_in, nzyr_inv, nzxr_inv, f, b1, b2);
+  recompose(c, b1, b2, a1q, a2q, k & 3, f, true, A1_out, A2_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -469,37 +494,38 @@ int acetn_hip_norm_tensor(const acetn_site* s1, const acetn_site* s2, int k, con
   });
 }
 
-size_t acetn_hip_apply_gate_workspace(int64_t D, int64_t d) {
+size_t acetn_hip_apply_gate_workspace(int64_t nD, int64_t D, int64_t d) {
   size_t r = 0;
   guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     r = measure([&](Ctx& c) { apply_gate(c, nullptr, nullptr, nullptr, f, nullptr); });
   });
   return r;
 }
 
-int acetn_hip_apply_gate(const double* a1r, const double* a2r, const double* gate, int64_t D,
-                         int64_t d, double* a12g, void* ws, size_t ws_bytes, void* stream) {
+int acetn_hip_apply_gate(const double* a1r, const double* a2r, const double* gate, int64_t nD,
+                         int64_t D, int64_t d, double* a12g, void* ws, size_t ws_bytes,
+                         void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     ACETN_REQUIRE(a1r && a2r && gate && a12g, "apply_gate: null pointer");
     execute(ws, ws_bytes, stream, [&](Ctx& c) { apply_gate(c, a1r, a2r, gate, f, a12g); });
   });
 }
 
-size_t acetn_hip_positive_approx_workspace(int64_t D, int64_t d) {
+size_t acetn_hip_positive_approx_workspace(int64_t nD) {
   size_t r = 0;
   guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, 1, 1);  // only nD enters
     r = measure([&](Ctx& c) { positive_approx(c, nullptr, f, 0.0, nullptr, nullptr); });
   });
   return r;
 }
 
-int acetn_hip_positive_approx(const double* n12, int64_t D, int64_t d, double cutoff, double* nz,
+int acetn_hip_positive_approx(const double* n12, int64_t nD, double cutoff, double* nz,
                               int32_t* host_shifts, void* ws, size_t ws_bytes, void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, 1, 1);
     ACETN_REQUIRE(n12 && nz, "positive_approx: null pointer");
     int shifts = 0;
     execute(ws, ws_bytes, stream, [&](Ctx& c) { positive_approx(c, n12, f, cutoff, nz, &shifts); });
@@ -507,10 +533,10 @@ int acetn_hip_positive_approx(const double* n12, int64_t D, int64_t d, double cu
   });
 }
 
-size_t acetn_hip_gauge_fix_workspace(int64_t D, int64_t d) {
+size_t acetn_hip_gauge_fix_workspace(int64_t nD, int64_t d) {
   size_t r = 0;
   guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, 1, d);  // D does not enter
     r = measure([&](Ctx& c) {
       gauge_fix(c, nullptr, nullptr, f, 0.0, nullptr, nullptr, nullptr, nullptr);
     });
@@ -518,11 +544,11 @@ size_t acetn_hip_gauge_fix_workspace(int64_t D, int64_t d) {
   return r;
 }
 
-int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t D, int64_t d,
+int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t nD, int64_t d,
                         double atol, double* n12, double* a12g, double* nzxr_inv,
                         double* nzyr_inv, void* ws, size_t ws_bytes, void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, 1, d);
     ACETN_REQUIRE(nz && a12g_in && n12 && a12g && nzxr_inv && nzyr_inv, "gauge_fix: null pointer");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       gauge_fix(c, nz, a12g_in, f, atol, n12, a12g, nzxr_inv, nzyr_inv);
@@ -530,12 +556,15 @@ int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t D, int6
   });
 }
 
-size_t acetn_hip_als_workspace(int64_t D, int64_t d) {
+size_t acetn_hip_als_workspace(int64_t nD, int64_t D, int64_t d) {
   size_t r = 0;
   guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
+    als_check_shape((int)f.nD, (int)f.D, (int)f.d, 0);
     r = measure([&](Ctx& c) {
-      plain_start(c, nullptr, f, nullptr, nullptr);
+      // the plain start where it exists (acetn_hip_als_initialize reports
+      // the shapes it cannot start from)
+      if (f.nD * f.d <= kSvdMaxN && f.D <= f.nD * f.d) plain_start(c, nullptr, f, nullptr, nullptr);
       c.take(als_work_elems((int)f.nD, (int)f.D, (int)f.d));
       c.take(2);
       c.take_int(4);
@@ -544,25 +573,26 @@ size_t acetn_hip_als_workspace(int64_t D, int64_t d) {
   return r;
 }
 
-int acetn_hip_als_initialize(const double* a12g, int64_t D, int64_t d, double* a1r, double* a2r,
-                             void* ws, size_t ws_bytes, void* stream) {
+int acetn_hip_als_initialize(const double* a12g, int64_t nD, int64_t D, int64_t d, double* a1r,
+                             double* a2r, void* ws, size_t ws_bytes, void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     ACETN_REQUIRE(a12g && a1r && a2r, "als_initialize: null pointer");
     execute(ws, ws_bytes, stream, [&](Ctx& c) { plain_start(c, a12g, f, a1r, a2r); });
   });
 }
 
 int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, double* a2r,
-                        int64_t D, int64_t d, int niter, double tol, int method, double epsilon,
-                        int64_t* host_sweeps, double* host_cost, int32_t* host_info, void* ws,
-                        size_t ws_bytes, void* stream) {
+                        int64_t nD, int64_t D, int64_t d, int niter, double tol, int method,
+                        double epsilon, int64_t* host_sweeps, double* host_cost,
+                        int32_t* host_info, void* ws, size_t ws_bytes, void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     ACETN_REQUIRE(n12 && a12g && a1r && a2r, "als_solve: null pointer");
     ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY || method == ACETN_HIP_ALS_PINV,
                   "als_solve: unknown method");
     ACETN_REQUIRE(niter >= 0, "als_solve: negative niter");
+    als_check_shape((int)f.nD, (int)f.D, (int)f.d, method == ACETN_HIP_ALS_PINV ? 1 : 0);
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       double* gw = c.take(als_work_elems((int)f.nD, (int)f.D, (int)f.d));
       double* cost = c.take(2);
@@ -582,10 +612,10 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
   });
 }
 
-size_t acetn_hip_als_batch_workspace(int64_t count, int64_t D, int64_t d) {
+size_t acetn_hip_als_batch_workspace(int64_t count, int64_t nD, int64_t D, int64_t d) {
   size_t r = 0;
   guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     ACETN_REQUIRE(count >= 1 && count <= (int64_t(1) << 24), "als_batch: count out of range");
     const int64_t n = f.nD * f.d;
     r = measure([&](Ctx& c) {
@@ -597,17 +627,20 @@ size_t acetn_hip_als_batch_workspace(int64_t count, int64_t D, int64_t d) {
 }
 
 int acetn_hip_als_solve_batch(const double* n12, const double* a12g, double* a1r, double* a2r,
-                              int64_t count, int64_t D, int64_t d, int niter, double tol,
-                              int method, double epsilon, int plain_start, double* dev_cost,
-                              int32_t* dev_info, void* ws, size_t ws_bytes, void* stream) {
+                              int64_t count, int64_t nD, int64_t D, int64_t d, int niter,
+                              double tol, int method, double epsilon, int plain_start,
+                              double* dev_cost, int32_t* dev_info, void* ws, size_t ws_bytes,
+                              void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     ACETN_REQUIRE(n12 && a12g && a1r && a2r && dev_cost && dev_info, "als_batch: null pointer");
     ACETN_REQUIRE(count >= 1 && count <= (int64_t(1) << 24), "als_batch: count out of range");
     ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY || method == ACETN_HIP_ALS_PINV,
                   "als_batch: unknown method");
     ACETN_REQUIRE(niter >= 0, "als_batch: negative niter");
     const int64_t n = f.nD * f.d;
+    ACETN_REQUIRE(!plain_start || (n <= kSvdMaxN && f.D <= n),
+                  "als_batch: the plain start needs nD d <= 32 and D <= nD d");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       double* M = c.take((size_t)count * n * n);
       double* U = c.take((size_t)count * n * n);
@@ -633,11 +666,11 @@ int acetn_hip_als_solve_batch(const double* n12, const double* a12g, double* a1r
 }
 
 int acetn_hip_als_solve_noalloc(const double* n12, const double* a12g, double* a1r, double* a2r,
-                                int64_t D, int64_t d, int niter, double tol, int method,
-                                double epsilon, int64_t* host_sweeps, double* host_cost,
-                                int32_t* host_info, void* stream) {
+                                int64_t nD, int64_t D, int64_t d, int niter, double tol,
+                                int method, double epsilon, int64_t* host_sweeps,
+                                double* host_cost, int32_t* host_info, void* stream) {
   return guarded([&] {
-    const FuDims f = fu_dims(D, d);
+    const FuDims f = solver_dims(nD, D, d);
     ACETN_REQUIRE(n12 && a12g && a1r && a2r, "als_solve_noalloc: null pointer");
     ACETN_REQUIRE(method == ACETN_HIP_ALS_CHOLESKY,
                   "als_solve_noalloc: only the 'cholesky' method (with its LU fallback) runs on "
@@ -688,6 +721,55 @@ int acetn_hip_finalize_bond(const double* a1r, const double* a2r, const double* 
                   "finalize_bond: give both gauge inverses or neither");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
       finalize_bond(c, a1r, a2r, nzyr_inv, nzxr_inv, a1q, a2q, k, f, A1_out, A2_out);
+    });
+  });
+}
+
+size_t acetn_hip_finalize_reduced_workspace(int64_t nD, int64_t D, int64_t d) {
+  size_t r = 0;
+  guarded([&] {
+    const FuDims f = solver_dims(nD, D, d);
+    r = measure([&](Ctx& c) {
+      finalize_reduced(c, nullptr, nullptr, nullptr, nullptr, f, nullptr, nullptr);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_finalize_reduced(const double* a1r, const double* a2r, const double* nzyr_inv,
+                               const double* nzxr_inv, int64_t nD, int64_t D, int64_t d,
+                               double* a1r_out, double* a2r_out, void* ws, size_t ws_bytes,
+                               void* stream) {
+  return guarded([&] {
+    const FuDims f = solver_dims(nD, D, d);
+    ACETN_REQUIRE(a1r && a2r && a1r_out && a2r_out, "finalize_reduced: null pointer");
+    ACETN_REQUIRE((nzyr_inv == nullptr) == (nzxr_inv == nullptr),
+                  "finalize_reduced: give both gauge inverses or neither");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      finalize_reduced(c, a1r, a2r, nzyr_inv, nzxr_inv, f, a1r_out, a2r_out);
+    });
+  });
+}
+
+size_t acetn_hip_recompose_workspace(int64_t D, int64_t d) {
+  size_t r = 0;
+  guarded([&] {
+    const FuDims f = fu_dims(D, d);
+    r = measure([&](Ctx& c) {
+      recompose(c, nullptr, nullptr, nullptr, nullptr, 0, f, false, nullptr, nullptr);
+    });
+  });
+  return r;
+}
+
+int acetn_hip_recompose(const double* a1r, const double* a2r, const double* a1q,
+                        const double* a2q, int k, int normalize, int64_t D, int64_t d,
+                        double* A1_out, double* A2_out, void* ws, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    const FuDims f = fu_dims(D, d);
+    ACETN_REQUIRE(a1r && a2r && a1q && a2q && A1_out && A2_out, "recompose: null pointer");
+    execute(ws, ws_bytes, stream, [&](Ctx& c) {
+      recompose(c, a1r, a2r, a1q, a2q, k & 3, f, normalize != 0, A1_out, A2_out);
     });
   });
 }

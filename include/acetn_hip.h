@@ -304,24 +304,31 @@ int acetn_hip_norm_tensor(const acetn_site* s1, const acetn_site* s2, int k, con
                           const double* a2q, double* n12, void* workspace, size_t ws_bytes,
                           void* stream);
 
-/* Gate application of FullUpdater.update_reduced_tensors,
+/* The bond solver's entry points below take the reduced tensors' shape
+ * (nD, D, d) as ALSSolver's ar_shape = (nD, bD, pD) gives it
+ * (als_solver.py:10-32), with any reduced dimension 1 <= nD <= 32: a
+ * full-update step has nD = min(D^3, D d).
+ *
+ * Gate application of FullUpdater.update_reduced_tensors,
  * full_update.py:77-86: a12g[y][x][r][s] = sum a1r[y,u,p] a2r[x,u,q] gate[p,q,r,s]. */
-size_t acetn_hip_apply_gate_workspace(int64_t D, int64_t d);
-int acetn_hip_apply_gate(const double* a1r, const double* a2r, const double* gate, int64_t D,
-                         int64_t d, double* a12g, void* workspace, size_t ws_bytes, void* stream);
+size_t acetn_hip_apply_gate_workspace(int64_t nD, int64_t D, int64_t d);
+int acetn_hip_apply_gate(const double* a1r, const double* a2r, const double* gate, int64_t nD,
+                         int64_t D, int64_t d, double* a12g, void* workspace, size_t ws_bytes,
+                         void* stream);
 
-/* positive_approx, full_update.py:262-297: nz (nD, nD, nD^2).  Synchronises
- * the stream once per eigen-decomposition (the reference's loop condition);
- * host_shifts receives the number of diagonal shifts applied. */
-size_t acetn_hip_positive_approx_workspace(int64_t D, int64_t d);
-int acetn_hip_positive_approx(const double* n12, int64_t D, int64_t d, double cutoff, double* nz,
+/* positive_approx, full_update.py:262-297: nz (nD, nD, nD^2) from n12
+ * (nD, nD, nD, nD), nD^2 <= 510.  Synchronises the stream once per
+ * eigen-decomposition (the reference's loop condition); host_shifts
+ * receives the number of diagonal shifts applied. */
+size_t acetn_hip_positive_approx_workspace(int64_t nD);
+int acetn_hip_positive_approx(const double* n12, int64_t nD, double cutoff, double* nz,
                               int32_t* host_shifts, void* workspace, size_t ws_bytes,
                               void* stream);
 
 /* gauge_fix, full_update.py:300-343: from nz and a12g, the gauge-fixed n12
  * and a12g and the pseudo-inverses (absolute cutoff atol) of the R factors. */
-size_t acetn_hip_gauge_fix_workspace(int64_t D, int64_t d);
-int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t D, int64_t d,
+size_t acetn_hip_gauge_fix_workspace(int64_t nD, int64_t d);
+int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t nD, int64_t d,
                         double atol, double* n12, double* a12g, double* nzxr_inv,
                         double* nzyr_inv, void* workspace, size_t ws_bytes, void* stream);
 
@@ -333,15 +340,18 @@ int acetn_hip_gauge_fix(const double* nz, const double* a12g_in, int64_t D, int6
  * sweep loop and its stopping rule run on the device (one workgroup);
  * host_sweeps/host_cost/host_info receive the sweep count, the final cost
  * and flags (1: a Cholesky factorisation failed and LU with partial pivoting
- * was used, as the reference's fallback; 2: a singular system). */
+ * was used, as the reference's fallback; 2: a singular system).  The final
+ * cost is ALSSolver.calculate_cost (als_solver.py:251-261) of the result;
+ * with niter = 0, of the start.  The plain start needs nD d <= 32 and
+ * D <= nD d. */
 enum { ACETN_HIP_ALS_CHOLESKY = 0, ACETN_HIP_ALS_PINV = 1 };
-size_t acetn_hip_als_workspace(int64_t D, int64_t d);
-int acetn_hip_als_initialize(const double* a12g, int64_t D, int64_t d, double* a1r, double* a2r,
-                             void* workspace, size_t ws_bytes, void* stream);
+size_t acetn_hip_als_workspace(int64_t nD, int64_t D, int64_t d);
+int acetn_hip_als_initialize(const double* a12g, int64_t nD, int64_t D, int64_t d, double* a1r,
+                             double* a2r, void* workspace, size_t ws_bytes, void* stream);
 int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, double* a2r,
-                        int64_t D, int64_t d, int niter, double tol, int method, double epsilon,
-                        int64_t* host_sweeps, double* host_cost, int32_t* host_info,
-                        void* workspace, size_t ws_bytes, void* stream);
+                        int64_t nD, int64_t D, int64_t d, int niter, double tol, int method,
+                        double epsilon, int64_t* host_sweeps, double* host_cost,
+                        int32_t* host_info, void* workspace, size_t ws_bytes, void* stream);
 
 /* ``count`` independent ALS problems in one launch (one workgroup per
  * problem; the ALS loop of als_solver.py:55-82 in each), for throughput work
@@ -353,11 +363,12 @@ int acetn_hip_als_solve(const double* n12, const double* a12g, double* a1r, doub
  * else they hold the starts.  Results stay on the device: dev_cost[b] the
  * final cost, dev_info[2b] the sweep count, dev_info[2b+1] the flags of
  * acetn_hip_als_solve.  Nothing synchronises. */
-size_t acetn_hip_als_batch_workspace(int64_t count, int64_t D, int64_t d);
+size_t acetn_hip_als_batch_workspace(int64_t count, int64_t nD, int64_t D, int64_t d);
 int acetn_hip_als_solve_batch(const double* n12, const double* a12g, double* a1r, double* a2r,
-                              int64_t count, int64_t D, int64_t d, int niter, double tol,
-                              int method, double epsilon, int plain_start, double* dev_cost,
-                              int32_t* dev_info, void* workspace, size_t ws_bytes, void* stream);
+                              int64_t count, int64_t nD, int64_t D, int64_t d, int niter,
+                              double tol, int method, double epsilon, int plain_start,
+                              double* dev_cost, int32_t* dev_info, void* workspace,
+                              size_t ws_bytes, void* stream);
 
 /* acetn_hip_als_solve without a workspace argument, for callers that cannot
  * allocate device memory (code under stream capture, tight serving loops,
@@ -367,9 +378,9 @@ int acetn_hip_als_solve_batch(const double* n12, const double* a12g, double* a1r
  * larger bonds return an error.  Calls that overlap on one device share that
  * array and race: serialise them. */
 int acetn_hip_als_solve_noalloc(const double* n12, const double* a12g, double* a1r, double* a2r,
-                                int64_t D, int64_t d, int niter, double tol, int method,
-                                double epsilon, int64_t* host_sweeps, double* host_cost,
-                                int32_t* host_info, void* stream);
+                                int64_t nD, int64_t D, int64_t d, int niter, double tol,
+                                int method, double epsilon, int64_t* host_sweeps,
+                                double* host_cost, int32_t* host_info, void* stream);
 
 /* finalize_reduced_tensors (full_update.py:123-161; nzyr_inv/nzxr_inv null
  * when the gauge fix is off) + recompose_site_tensors (tensor_update.py:70-75),
@@ -380,6 +391,26 @@ int acetn_hip_finalize_bond(const double* a1r, const double* a2r, const double* 
                             const double* nzxr_inv, const double* a1q, const double* a2q, int k,
                             int64_t D, int64_t d, double* A1_out, double* A2_out, void* workspace,
                             size_t ws_bytes, void* stream);
+
+/* The two halves of acetn_hip_finalize_bond on their own.  _finalize_reduced
+ * is FullUpdater.finalize_reduced_tensors (full_update.py:123-161) on
+ * reduced tensors (nD, D, d), any nD with D <= nD d and D <= 32: a1r_out,
+ * a2r_out (nD, D, d), with the bond-index sign rule of _finalize_bond.
+ * _recompose is TensorUpdater.recompose_site_tensors (tensor_update.py:70-75)
+ * from a1q/a2q (D,D,D,D d) and a1r/a2r (D d, D, d); with normalize != 0 each
+ * tensor is divided by its norm (full_update.py:61-63); k rotates the result
+ * from the bond frame back to the site frame (tensor_update.py:30-32; k = 0
+ * leaves it in the bond frame). */
+size_t acetn_hip_finalize_reduced_workspace(int64_t nD, int64_t D, int64_t d);
+int acetn_hip_finalize_reduced(const double* a1r, const double* a2r, const double* nzyr_inv,
+                               const double* nzxr_inv, int64_t nD, int64_t D, int64_t d,
+                               double* a1r_out, double* a2r_out, void* workspace,
+                               size_t ws_bytes, void* stream);
+size_t acetn_hip_recompose_workspace(int64_t D, int64_t d);
+int acetn_hip_recompose(const double* a1r, const double* a2r, const double* a1q,
+                        const double* a2q, int k, int normalize, int64_t D, int64_t d,
+                        double* A1_out, double* A2_out, void* workspace, size_t ws_bytes,
+                        void* stream);
 
 /* SimpleUpdater.bond_update, acetn/evolution/simple_update.py:28-76, with
  * LambdaTensors.absorb/strip (lambda_tensors.py:34-49): lam1[leg]/lam2[leg]

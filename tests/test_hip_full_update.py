@@ -100,7 +100,7 @@ def test_positive_approx_matches_oracle(hip):
         # shift_case 1 makes the matrix indefinite so that the shift loop runs
         N = n12 - shift_case * 1e-3 * np.abs(n12).max() * np.eye(nD * nD).reshape(n12.shape)
         ref = ofu.positive_approx(N, 1e-12)
-        nz, shifts = hip.positive_approx(dev(N), D, 2, 1e-12)
+        nz, shifts = hip.positive_approx(dev(N), 1e-12)
         nz = host(nz)
         # nz is fixed up to an orthogonal mixing of its last index
         p_ref = ein("xyz,XYz->xyXY", ref, ref)
@@ -118,7 +118,7 @@ def test_gauge_fix_matches_oracle(hip):
     a12g = ein("yxpq,pqrs->yxrs", ein("yup,xuq->yxpq", a1r, a2r), gate)
     nz = ofu.positive_approx(n12, 1e-12)
     rn12, ra12g, rxi, ryi = ofu.gauge_fix(nz, a12g, 1e-12)
-    hn12, ha12g, hxi, hyi = [host(t) for t in hip.gauge_fix(dev(nz), dev(a12g), D, 2, 1e-12)]
+    hn12, ha12g, hxi, hyi = [host(t) for t in hip.gauge_fix(dev(nz), dev(a12g), 1e-12)]
     # the R factors are fixed up to the signs of their rows: compare the
     # gauge-fixed problem through invariants of that sign freedom
     for r, h in ((rxi, hxi), (ryi, hyi)):
@@ -202,8 +202,8 @@ def hip_tensor_update(hip, st_dev, bond, gate, use_gauge_fix=True):
     a1q, a1r, a2q, a2r = hip.bond_decompose(A1, A2, k)
     n12 = hip.norm_tensor(st_dev[s1], st_dev[s2], k, a1q, a2q)
     a12g = hip.apply_gate(a1r, a2r, dev(gate))
-    nz, _ = hip.positive_approx(n12, A1.shape[0], A1.shape[4], 1e-12)
-    n12, a12g, gx, gy = hip.gauge_fix(nz, a12g, A1.shape[0], A1.shape[4], 1e-12)
+    nz, _ = hip.positive_approx(n12, 1e-12)
+    n12, a12g, gx, gy = hip.gauge_fix(nz, a12g, 1e-12)
     b1, b2 = hip.als_initialize(a12g, A1.shape[0])
     b1, b2, sweeps, cost, _ = hip.als_solve(n12, a12g, b1, b2, 100, 1e-15, "cholesky", 1e-12)
     return hip.finalize_bond(b1, b2, gy, gx, a1q, a2q, k), sweeps

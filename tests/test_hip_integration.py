@@ -63,10 +63,15 @@ def test_integration_adapter_reports_errors(adapter):
     start = ofu.initialize_reduced_tensors(a12g, shape)
     with pytest.raises(KeyError):
         adapter(dev(start[0]), dev(start[1]), None, dev(n12), dev(a12g), 10, 1e-12, "qr", 1e-12)
-    D, d = 9, 2  # the device full update serves D <= 8
+    D, d = 9, 2  # the one-workgroup solver holds systems up to about nD D = 140
     nD = D * d
     rng = np.random.default_rng(5)
     a1r = dev(rng.standard_normal((nD, D, d)))
-    with pytest.raises(RuntimeError, match="unsupported D"):
+    with pytest.raises(RuntimeError, match="too large for one workgroup"):
         adapter(a1r, a1r.clone(), None, dev(rng.standard_normal((nD,) * 4)),
                 dev(rng.standard_normal((nD, nD, d, d))), 10, 1e-12, "cholesky", 1e-12)
+    nD = 33  # beyond the reduced dimensions the library accepts
+    a1r = dev(rng.standard_normal((nD, 2, d)))
+    with pytest.raises(RuntimeError, match="unsupported reduced tensor shape"):
+        adapter(a1r, a1r.clone(), None, dev(np.zeros((nD,) * 4)),
+                dev(np.zeros((nD, nD, d, d))), 10, 1e-12, "cholesky", 1e-12)

@@ -47,5 +47,10 @@ def test_noalloc_matches_workspace_entry_point(hip):
 def test_noalloc_refuses_bonds_beyond_its_scratch(hip):
     n12, a12g = problem(3, 2, 3)
     big = torch.zeros(32, 8, 4, dtype=torch.float64, device="cuda")  # D = 8, d = 4: nD * D = 256
-    with pytest.raises(RuntimeError, match="at most 128"):
+    # operands of mismatched shapes never reach the library
+    with pytest.raises(ValueError, match="shape"):
         hip.als_solve_noalloc(n12, a12g, big, big.clone(), 10, 1e-12, "cholesky", 1e-12)
+    n12b = torch.zeros((32,) * 4, dtype=torch.float64, device="cuda")
+    a12gb = torch.zeros(32, 32, 4, 4, dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="at most 128"):
+        hip.als_solve_noalloc(n12b, a12gb, big, big.clone(), 10, 1e-12, "cholesky", 1e-12)

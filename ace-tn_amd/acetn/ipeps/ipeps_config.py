@@ -104,13 +104,21 @@ class IpepsConfig:
             self.evolution.disable_progressbar = not interactive
 
     def set_dtype(self, dtype):
-        self.dtype = torch.float64 if dtype is None else dtype
-        if isinstance(dtype, str):
-            self.dtype = getattr(torch, dtype)
-        if self.dtype != torch.float64:
+        """ipeps_config.py:86-90.  The engine stores and computes in float64:
+        a real dtype of lower precision (the reference's float32) is noted
+        and raised to float64 (lower precision is chosen per stage, with
+        ctmrg.precision); complex dtypes are not served."""
+        d = torch.float64 if dtype is None else dtype
+        if isinstance(d, str):
+            d = getattr(torch, d)
+        if d.is_complex:
             raise NotImplementedError(
-                f"dtype={self.dtype}: the MI355X engine stores tensors in float64; lower "
-                "precision is selected per stage with ctmrg.precision")
+                f"dtype={d}: the MI355X engine serves real networks only (float64)")
+        if d != torch.float64:
+            tqdm.write(f"acetn (MI355X engine): dtype={d} computed and stored as float64; "
+                       "lower precision is selected per stage with ctmrg.precision")
+            d = torch.float64
+        self.dtype = d
 
     def set_device(self, device):
         """The engine keeps every tensor on the GPU.  A 'cpu' request (the

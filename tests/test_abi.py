@@ -39,6 +39,7 @@ def test_hip_backend_refuses_without_gpu():
 
 
 def test_config_validation():
+    import torch
     from acetn.ipeps.ipeps_config import IpepsConfig
     cfg = IpepsConfig(TN={'nx': 2, 'ny': 2, 'dims': {'phys': 2, 'bond': 2, 'chi': 8}},
                       evolution={'backend': 'cutensor'}, device="cpu")
@@ -51,8 +52,10 @@ def test_config_validation():
         IpepsConfig(TN={'nx': 2, 'ny': 2}, evolution={'backend': 'numpy'})
     with pytest.raises(TypeError):
         IpepsConfig(TN={'nx': 2, 'ny': 2}, ctmrg={'no_such_key': 1})
-    with pytest.raises(NotImplementedError):
-        IpepsConfig(TN={'nx': 2, 'ny': 2}, dtype="float32")
+    assert IpepsConfig(TN={'nx': 2, 'ny': 2}, dtype="float32").dtype == torch.float64
+    assert IpepsConfig(TN={'nx': 2, 'ny': 2}, dtype=torch.float32).dtype == torch.float64
+    with pytest.raises(NotImplementedError, match="real"):
+        IpepsConfig(TN={'nx': 2, 'ny': 2}, dtype="complex128")
 
 
 def test_ctmrg_graph_config_raises_while_switched_off():

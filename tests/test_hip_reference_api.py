@@ -118,6 +118,27 @@ def test_als_general_reduced_dimension_matches_oracle():
     np.testing.assert_allclose(npy(h12g), n12g, rtol=0, atol=1e-12 * np.abs(n12g).max())
 
 
+def test_als_float32_operands():
+    """The reference's float32 ALS case (tests/unit/test_als_cutensor.py:137-140,
+    cost within 5 % of the torch backend's): float32 operands are solved in
+    float64 and the results come back in float32; their cost equals the
+    oracle's float64 solve of the same (float32-valued) operands."""
+    from acetn.evolution.als_solver import ALSSolver
+    from acetn.ipeps.ipeps_config import EvolutionConfig
+    c = als_case(seed=8)
+    n12 = c['n12'].to(torch.float32).cuda()
+    a12g = c['a12g'].to(torch.float32).cuda()
+    solver = ALSSolver(n12, a12g, c['shape'], EvolutionConfig(als_niter=50, als_tol=1e-15))
+    a1r, a2r = solver.solve()
+    assert a1r.dtype == torch.float32 and a1r.device.type == "cuda"
+    on12, oa12g = npy(n12).astype(np.float64), npy(a12g).astype(np.float64)
+    r1, r2, sweeps, cost = ofu.als_solve(on12, oa12g, c['shape'], 50, 1e-15, "cholesky", 1e-12)
+    assert solver.sweeps == sweeps
+    assert solver.cost == pytest.approx(cost, rel=1e-9)
+    c32 = float(ALSSolver.calculate_cost(a1r, a2r, a12g, n12))
+    assert c32 == pytest.approx(cost, rel=1e-5)   # the float32 rounding of the results
+
+
 # ---- full update (tests/integration/test_full_update.py) -------------------
 
 class FuConfig:

@@ -251,6 +251,33 @@ def test_mixed_ctmrg_energy_matches_oracle(hip, D, chi):
     assert abs(e_h - e_o) <= MIXED_DE
 
 
+def test_mixed_ctmrg_from_default_start(hip):
+    """Mixed precision from the default start at D=6 chi=36 (graded
+    spectra): three iterations keep every boundary shape of the FP64 oracle
+    (same sketches) and its energy per site to |dE| <= 1e-6."""
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+    D, chi = 6, 36
+    st0 = product_start(D, 33)
+    seed = 616
+    ost = copy.deepcopy(st0)
+    omover = oc.DirectionalMover(oc.ProjectorCalculator(omega_fn=SketchStream(seed)), 2, 2, chi)
+    hst = DevState(st0, 2, 2, D, chi, seed)
+    hmover = DirectionalMover(MixedCfg(), hst.sketch)
+    for _ in range(3):
+        oc.ctmrg_iteration(omover, ost)
+        ctmrg_iteration(hmover, hst)
+    h = hst.to_numpy()
+    for s in h:
+        for key in ('C', 'E'):
+            for kk in range(4):
+                assert h[s][key][kk].shape == ost[s][key][kk].shape, (s, key, kk)
+    e_o = om.measure(ost, 2, 2, omod.Heisenberg(1.0))['Energy']
+    e_h = om.measure(h, 2, 2, omod.Heisenberg(1.0))['Energy']
+    print(f"default start D=6 chi=36: mixed {e_h:.15f} oracle {e_o:.15f} dE {abs(e_h - e_o):.2e}")
+    assert abs(e_h - e_o) <= MIXED_DE
+
+
 @pytest.mark.parametrize("case", list(GOLDEN_CASES))
 def test_mixed_ctmrg_on_golden_fixture(hip, case):
     from acetn.renormalization.ctmrg import ctmrg_iteration

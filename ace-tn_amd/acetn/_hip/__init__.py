@@ -404,6 +404,9 @@ def kernel_timing(enable):
 K_GEMM_F64, K_GEMM_SPLIT, K_DOUBLE_LAYER, K_JACOBI, K_CHOLESKY = 0, 1, 2, 3, 4
 # precision modes (ACETN_HIP_FP64 / ACETN_HIP_MIXED)
 PRECISIONS = {"fp64": 0, "mixed": 1}
+# OR-ed into a projector pair's precision code: the range finder refills
+# dropped columns at random (cheaper; identical unless the spectrum is graded)
+RF_FAST = 0x100
 
 
 def precision_code(precision):
@@ -591,7 +594,7 @@ class ProjectorJob:
 
     def __init__(self, slot, sites, k, chi, oversampling, niter, cutoff, seed, stream_id,
                  precision="fp64", system="half-system", svd="rsvd", ws_key=None, graph=None,
-                 mark=False):
+                 mark=False, rf_fast=False):
         """With ``graph`` = (stream_base, counts, index) the pair is enqueued in
         the capturable form (acetn_hip_projectors_begin_graph / _end_padded):
         the sketch stream is stream_id + stream_base[0] (an int64 device
@@ -600,14 +603,18 @@ class ProjectorJob:
         and ``finish`` returns min(chi, rows) columns with those past the
         count zero.  With ``mark`` the event ``self.mark`` is recorded on the
         pair's stream when its range-finder products are enqueued, ahead of
-        the eigensolver (acetn_hip_projectors_begin_mark)."""
+        the eigensolver (acetn_hip_projectors_begin_mark).  ``rf_fast`` sets
+        RF_FAST (for pairs whose previous spectrum was not graded); the
+        ungraded test reads the spectrum: ``self.spec_host`` receives its q
+        computed values on the host by the time ``ready`` completes."""
         made = [make_site(*st) for st in sites]
         self.sv = [m[0] for m in made]
         self.keep = [m[1] for m in made]
         self.arr = (ctypes.POINTER(AcetnSite) * 4)(*[ctypes.pointer(x) for x in self.sv])
         self.k, self.chi, self.p, self.niter = k, chi, oversampling, niter
-        self.prec = precision_code(precision)
+        self.prec = precision_code(precision) | (RF_FAST if rf_fast else 0)
         self.system, self.svd = SYSTEMS[system], SVD_TYPES[svd]
+        self.spec_host = None
         s1 = self.sv[0]
         self.dev = self.keep[0][0].device
         self.rows = s1.E_dim[(3 + k) % 4][0] * s1.D * s1.D
@@ -655,7 +662,12 @@ class ProjectorJob:
                     self.arr, k, chi, oversampling, niter, cutoff, seed, stream_id, self.system,
                     self.svd, self.prec, c_vp(self.cd_host.data_ptr()), ptr(self.spec),
                     ptr(self.ws), self.ws.numel(), c_vp(self.stream.cuda_stream)))
-            # marks the truncation count's arrival in pinned host memory
+            if self.svd == SVD_TYPES["rsvd"]:
+                q = chi + oversampling
+                self.spec_host = torch.empty(q, dtype=torch.float64, pin_memory=True)
+                self.spec_host.copy_(self.spec[:q], non_blocking=True)
+            # marks the truncation count's (and spectrum's) arrival in pinned
+            # host memory
             self.ready = torch.cuda.Event()
             self.ready.record(self.stream)
 

@@ -7,6 +7,10 @@ import collections
 from .. import _hip
 
 
+# s_{q-1} / s_0 at or above which a pair's spectrum counts as not graded
+UNGRADED = 1e-3
+
+
 class SketchCounter:
     """Numbers the rSVD calls: call i uses Philox stream (first + i) under
     ``seed``; the reference draws a fresh torch.randn sketch per call
@@ -36,6 +40,9 @@ class ProjectorCalculator:
         # (spectrum, kept count) of the most recent projector pairs, newest
         # last: the singular values the truncation chooses from
         self.recent_spectra = collections.deque(maxlen=256)
+        # pair (sites, k, system) -> its last spectrum was not graded, so the
+        # next computation may take the range finder's fast path (RF_FAST)
+        self.ungraded = {}
         self.set_calculate()
 
     def set_calculate(self):
@@ -98,12 +105,16 @@ class ProjectorCalculator:
                 stream_id, graph = capture.next_pair(self)
             else:
                 stream_id = self.sketch.take() if self.svd_type == "rsvd" else 0
-            started.append(_hip.ProjectorJob(slot, [self._site(ipeps, s) for s in sites], k % 4,
-                                             chi, self.rsvd_oversampling, self.rsvd_niter,
-                                             self.svd_cutoff, self.sketch.seed, stream_id,
-                                             precision=self.precision, system=system,
-                                             svd=self.svd_type, ws_key=ws_key, graph=graph,
-                                             mark=mark))
+            key = (tuple(sites), k % 4, system)
+            job = _hip.ProjectorJob(slot, [self._site(ipeps, s) for s in sites], k % 4,
+                                    chi, self.rsvd_oversampling, self.rsvd_niter,
+                                    self.svd_cutoff, self.sketch.seed, stream_id,
+                                    precision=self.precision, system=system,
+                                    svd=self.svd_type, ws_key=ws_key, graph=graph,
+                                    mark=mark,
+                                    rf_fast=graph is None and self.ungraded.get(key, False))
+            job.key = key
+            started.append(job)
         return started
 
     def finish_each(self, ipeps, started, eager=False):
@@ -122,6 +133,11 @@ class ProjectorCalculator:
         D = ipeps.dims['bond']
 
         def finish(job):
+            if job.spec_host is not None:
+                # the range finder's CholeskyQR keeps every column when the
+                # q values span less than 1e-3 (margin 100 on its rank guard)
+                s = job.spec_host
+                self.ungraded[job.key] = bool(s[0] > 0 and s[-1] >= UNGRADED * s[0])
             P1, P2 = job.finish()
             # the kept count: a device scalar for a captured pair (written on
             # every replay)

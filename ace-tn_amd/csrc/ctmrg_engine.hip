@@ -149,6 +149,14 @@ const double* orthonormalize(Ctx& c, double* Y, int64_t m, int64_t q, bool steer
   if (single) return Yt;
   // pass 1: plain with rank guard (Yt -> Y); dropped columns come out zero
   pass(Yt, Y, 0.0, 1e-14, dead, nullptr, nullptr, false);
+  if (!c.recover) {
+    // ACETN_HIP_RF_FAST: dropped columns refilled at random, then the two
+    // plain passes (all skipped on the device when none was dropped)
+    if (c.launch) refill_dead_columns(Y, m, qi, dead, 0x243F6A8885A308D3ull, c.st);
+    pass(Y, Yt, 0.0, 1e-14, nullptr, dead, any, false);
+    pass(Yt, Y, 0.0, 1e-14, nullptr, dead, any, false);
+    return Y;
+  }
   // recovery levels; ``prev`` flags the columns still dropped (level 0: the
   // pass-1 ones), ``next`` receives those a level leaves dropped
   int* prev = dead;
@@ -1055,7 +1063,7 @@ int acetn_hip_quarter_tensor(const acetn_site* s, int k, int precision, double* 
     check_site(s);
     check_precision(precision);
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       quarter_tensor(c, s, k, Q);
     });
   });
@@ -1088,7 +1096,7 @@ int acetn_hip_half_projectors(const acetn_site* s1, const acetn_site* s4, int k,
     ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
     HalfOut o{P1, P2, host_cD_new, nullptr, 0, spectrum};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o);
     });
   });
@@ -1143,7 +1151,7 @@ int acetn_hip_half_projectors_begin(const acetn_site* s1, const acetn_site* s4, 
     ACETN_REQUIRE(niter >= 0 && niter <= 16, "half_projectors: niter out of range");
     HalfOut o{nullptr, nullptr, nullptr, pinned_cD_new, 0, spectrum};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       half_projectors(c, s1, s4, k, chi, oversampling, niter, svd_cutoff, seed, sketch_stream, o, 1);
     });
   });
@@ -1159,7 +1167,7 @@ int acetn_hip_half_projectors_end(const acetn_site* s1, const acetn_site* s4, in
     ACETN_REQUIRE(P1 && P2, "half_projectors_end: null output");
     HalfOut o{P1, P2, nullptr, nullptr, (int)cD_new, nullptr};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       half_projectors(c, s1, s4, k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
     });
   });
@@ -1205,7 +1213,7 @@ int acetn_hip_projectors_begin_mark(const acetn_site* const sites[4], int k, int
     HalfOut o{nullptr, nullptr, nullptr, pinned_cD_new, 0, spectrum};
     o.mark = reinterpret_cast<hipEvent_t>(mark_event);
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       if (system == kSysHalf && svd_type == kSvdRsvd)
         half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, svd_cutoff, seed,
                         sketch_stream, o, 1);
@@ -1226,7 +1234,7 @@ int acetn_hip_projectors_end(const acetn_site* const sites[4], int k, int64_t ch
     ACETN_REQUIRE(P1 && P2, "projectors_end: null output");
     HalfOut o{P1, P2, nullptr, nullptr, (int)cD_new, nullptr};
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       if (system == kSysHalf && svd_type == kSvdRsvd)
         half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
       else
@@ -1251,7 +1259,7 @@ int acetn_hip_projectors_begin_graph(const acetn_site* const sites[4], int k, in
     o.stream_base = dev_stream_base;
     o.dev_cD = dev_cD_new;
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       if (system == kSysHalf && svd_type == kSvdRsvd)
         half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, svd_cutoff, seed,
                         sketch_stream, o, 1);
@@ -1273,7 +1281,7 @@ int acetn_hip_projectors_end_padded(const acetn_site* const sites[4], int k, int
     HalfOut o{P1, P2, nullptr, nullptr, (int)cD_cols, nullptr};
     o.padded = true;
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       if (system == kSysHalf && svd_type == kSvdRsvd)
         half_projectors(c, sites[0], sites[3], k, chi, oversampling, niter, 0.0, 0, 0, o, 2);
       else
@@ -1306,7 +1314,7 @@ int acetn_hip_renormalize_boundary(const acetn_site* s1, int k, const double* P1
     ACETN_REQUIRE(cD_i >= 1 && cD_j >= 1, "renormalize: empty projector");
     ACETN_REQUIRE(P1_i && P2_i && P1_j && P2_j && C_a && C_b && E_o, "renormalize: null pointer");
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o);
     });
   });
@@ -1352,7 +1360,7 @@ int acetn_hip_renormalize_boundary_part(const acetn_site* s1, int k, int part, i
       ACETN_REQUIRE(ok2, "renormalize_part: null pointer");
     }
     execute(ws, ws_bytes, stream, [&](Ctx& c) {
-      c.mixed = precision == ACETN_HIP_MIXED;
+      set_precision(c, precision);
       renormalize(c, s1, k, P1_i, P2_i, cD_i, P1_j, P2_j, cD_j, C_a, C_b, E_o, part, cD_layout,
                   first);
     });

@@ -25,6 +25,7 @@ struct Ctx {
   size_t gwork_elems = 0;
   size_t gwork_need = 0;
   bool mixed = false;  // ACETN_HIP_MIXED: large products on split-bf16 matrix cores
+  bool recover = true; // range finder re-derives the columns CholeskyQR drops
   int split = 0;       // split level of the products enqueued now (0 = FP64)
 
   double* take(size_t n) {
@@ -126,7 +127,14 @@ inline GemmDesc gdesc(int M, int N, int K, IdxMap am, IdxMap ak, IdxMap bk, IdxM
 }
 
 inline void check_precision(int p) {
-  ACETN_REQUIRE(p == ACETN_HIP_FP64 || p == ACETN_HIP_MIXED, "unknown precision mode");
+  const int base = p & ~ACETN_HIP_RF_FAST;
+  ACETN_REQUIRE(base == ACETN_HIP_FP64 || base == ACETN_HIP_MIXED, "unknown precision mode");
+}
+// precision argument -> context: the arithmetic of the large products and
+// whether the range finder recovers dropped columns (ACETN_HIP_RF_FAST off)
+inline void set_precision(Ctx& c, int p) {
+  c.mixed = (p & ~ACETN_HIP_RF_FAST) == ACETN_HIP_MIXED;
+  c.recover = (p & ACETN_HIP_RF_FAST) == 0;
 }
 
 template <class F>

@@ -57,6 +57,15 @@ typedef struct acetn_site {
  * No reference counterpart: the reference runs one dtype end to end
  * (acetn/ipeps/ipeps_config.py:15, ``dtype``). */
 enum { ACETN_HIP_FP64 = 0, ACETN_HIP_MIXED = 1 };
+/* OR-ed into the precision argument of a projector entry point: the range
+ * finder's orthonormalisations refill the columns their CholeskyQR rank
+ * guard drops with random vectors instead of re-deriving them (some 30 fewer
+ * launches per orthonormalisation).  Identical results whenever no column is
+ * dropped, which holds when the spectrum is not graded: a caller can tell
+ * from the previous spectrum of the same pair (s_{q-1} / s_0 >= 1e-3 leaves a
+ * margin of 100).  On graded spectra (a product-state start) the refilled
+ * columns lose the small singular directions, so the default recovers them. */
+enum { ACETN_HIP_RF_FAST = 0x100 };
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* acetn_hip_last_error(void);

@@ -168,6 +168,27 @@ def test_half_projectors_on_graded_spectrum_match_oracle(hip, k):
     assert col.max() < 1e-7
 
 
+def test_range_finder_fast_path_is_identical_on_ungraded_spectra(hip):
+    """RF_FAST (dropped columns refilled at random) and the default recovery
+    give the same bits when the spectrum is not graded (no column dropped):
+    the projector calculator takes the fast path for such pairs."""
+    st = oc.make_state(oc.random_site_tensors(2, 2, 3, 2, seed=41))
+    oc.ctmrg(st, 2, 2, 16, 1, oc.ProjectorCalculator(omega_fn=SketchStream(9)))
+    sites = oc.plaquette(2, 2, 0, 1, 0)
+    dsites = [to_dev_site(st[s]) for s in sites]
+    out = []
+    for fast in (False, True):
+        job = hip.ProjectorJob(0, dsites, 0, 16, 2, 2, 1e-12, 21, 3, rf_fast=fast)
+        job.ready.synchronize()
+        P1, P2 = job.finish()
+        torch.cuda.synchronize()
+        out.append((P1.cpu(), P2.cpu(), job.spec.cpu(), job.spec_host.clone()))
+    s = out[0][3]
+    assert float(s[-1]) >= 1e-3 * float(s[0])     # not graded
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert torch.equal(a, b)
+
+
 def test_ctmrg_iterations_from_default_start_match_oracle(hip):
     """Three CTMRG iterations at D=6 chi=36 from the default start (graded
     spectra in every projector pair) against the oracle, same sketches.  For

@@ -35,6 +35,8 @@ class _Job:
         self.ready = _Ready(log, idx, arrived)
         self.graph = None
         self.spec = torch.zeros(4)
+        self.spec_host = None
+        self.key = idx
         self.stream = None
         self.log, self.idx = log, idx
 

@@ -168,6 +168,37 @@ def test_half_projectors_on_graded_spectrum_match_oracle(hip, k):
     assert col.max() < 1e-7
 
 
+def test_full_system_projectors_on_graded_spectrum_match_oracle(hip):
+    """Full-system rSVD (fused_3matmul_svd_lowrank, a basis orthonormalised
+    between the two halves of every power iteration) from the default start:
+    the product of four corners squares the graded half-system spectrum, so
+    the kept count falls below chi -- the same count, values and CTMRG step
+    as the oracle's."""
+    from acetn.renormalization.ctmrg import ctmrg_iteration
+    from acetn.renormalization.directional_mover import DirectionalMover
+
+    class FCfg(Cfg):
+        projectors = "full-system"
+    D, chi = 6, 36
+    st0 = product_start(D, 34)
+    seed = 717
+    ost = copy.deepcopy(st0)
+    ocalc = oc.ProjectorCalculator(projectors="full-system", omega_fn=SketchStream(seed))
+    omover = oc.DirectionalMover(ocalc, 2, 2, chi)
+    hst = DevState(st0, 2, 2, D, chi, seed)
+    hmover = DirectionalMover(FCfg(), hst.sketch)
+    oc.ctmrg_iteration(omover, ost)
+    ctmrg_iteration(hmover, hst)
+    h = hst.to_numpy()
+    shapes = {h[s]['C'][k].shape for s in h for k in range(4)}
+    print("full-system from the default start: corner shapes", sorted(shapes))
+    assert any(sh[0] < chi or sh[1] < chi for sh in shapes)     # truncated below chi
+    assert_state_close(h, ost, rtol=1e-7)
+    e_o = om.measure(ost, 2, 2, omod.Heisenberg(1.0))['Energy']
+    e_h = om.measure(h, 2, 2, omod.Heisenberg(1.0))['Energy']
+    assert e_h == pytest.approx(e_o, rel=1e-12)
+
+
 def test_range_finder_fast_path_is_identical_on_ungraded_spectra(hip):
     """RF_FAST (dropped columns refilled at random) and the default recovery
     give the same bits when the spectrum is not graded (no column dropped):
